@@ -1,0 +1,138 @@
+/*
+ * nfftm.h -- C ABI of libnfftm.so, an MI355X (gfx950) backend for the NFFT
+ * plan of JuliaMath/NFFT.jl.
+ *
+ * The entry points are the operations of the AbstractNFFTs plan interface that
+ * a Julia `ccall` (or Python ctypes) binding needs.  Each declaration names the
+ * reference interface it replaces.  Conventions:
+ *
+ *  - Arrays follow Julia's column-major layout: an image of size
+ *    N = (N_1, ..., N_D) is stored with dimension 1 fastest; the oversampled
+ *    grid (size Ntilde) likewise.  Complex data are interleaved (re, im).
+ *  - Node coordinates are a host D x M column-major double matrix (node j's
+ *    D coordinates are contiguous), as `k::Matrix` in
+ *    AbstractNFFTs/src/derived.jl:29-30.  Index arithmetic on nodes is FP64
+ *    in every precision.
+ *  - Data buffers (f, fHat, g) are DEVICE pointers owned by the caller, of
+ *    element type complex<float> (NFFTM_F32) or complex<double> (NFFTM_F64).
+ *  - Every call returns an nfftm_status; nfftm_last_error() gives the message
+ *    of the last failure on the calling thread.
+ *  - A plan is not re-entrant (it owns one grid buffer, like `p.tmpVec` in
+ *    src/implementation.jl:16-43); use nfftm_plan_copy for concurrency.  All
+ *    work is enqueued on the plan's HIP stream; the transform calls return
+ *    without synchronising unless NFFTM_FLAG_TIMING is set.
+ */
+#ifndef NFFTM_H
+#define NFFTM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nfftm_plan_s* nfftm_plan;
+
+/* Status codes; the Julia/Python binding maps them to the exception the
+ * reference raises for the same condition. */
+typedef enum {
+  NFFTM_OK = 0,
+  NFFTM_ERR_ARG = 1,         /* ArgumentError   (src/utils.jl:46-55, src/convolution.jl:52) */
+  NFFTM_ERR_DIM = 2,         /* DimensionMismatch (src/utils.jl:98-105, src/convolution.jl:26-29) */
+  NFFTM_ERR_UNSUPPORTED = 3, /* error(...)  (src/windowFunctions.jl:16, src/precomputation.jl:401) */
+  NFFTM_ERR_HIP = 4,         /* HIP runtime failure */
+  NFFTM_ERR_FFT = 5,         /* rocFFT failure */
+  NFFTM_ERR_OOM = 6          /* device allocation failed */
+} nfftm_status;
+
+/* Arithmetic of data, grid, spreading/interpolation and FFT.  Node index
+ * reduction and the deconvolution factors stay FP64 in both modes. */
+enum { NFFTM_F32 = 0, NFFTM_F64 = 1, NFFTM_MIXED = 2 /* alias of NFFTM_F32 */ };
+
+/* Precompute flags, same values as AbstractNFFTs/src/misc.jl:13-18. */
+enum { NFFTM_FULL = 1, NFFTM_TENSOR = 2, NFFTM_LINEAR = 3, NFFTM_POLYNOMIAL = 4 };
+
+/* Window ids (src/windowFunctions.jl:4-19).  Only Kaiser-Bessel is built. */
+enum { NFFTM_KAISER_BESSEL = 0 };
+
+/* Plan flags. */
+enum {
+  NFFTM_FLAG_BLOCKING = 1, /* tiled kernels (NFFTParams.blocking, default true) */
+  NFFTM_FLAG_TIMING = 2    /* record per-stage device times (TimingStats) */
+};
+
+/* Plan creation: replaces NFFTPlan(k, N; m, sigma, window, precompute,
+ * blocking) -- src/implementation.jl:73-106 with initParams
+ * src/precomputation.jl:3-56 -- as reached from
+ * plan_nfft(::Backend, ::Type{<:AbstractGPUArray}, k, N) at
+ * ext/NFFTGPUArraysExt/implementation.jl:21-59.  m and sigma are the
+ * resolved accuracy parameters (AbstractNFFTs/src/misc.jl:66-81 is applied
+ * by the caller).  hip_stream may be NULL (default stream). */
+int nfftm_plan_create(nfftm_plan* out, int prec, int D, const int64_t* N, int64_t M,
+                      const double* k_host, int m, double sigma, int window, int precompute,
+                      int flags, void* hip_stream);
+
+/* nodes!(p, k): src/implementation.jl:108-141 (AbstractNFFTs/src/interface.jl:211).
+ * M may differ from the plan's current M. */
+int nfftm_nodes(nfftm_plan p, const double* k_host, int64_t M);
+
+/* mul!(fHat, p, f): src/implementation.jl:155-172 (interface.jl:170).
+ * f: prod(N) complex, fhat: M complex. */
+int nfftm_forward(nfftm_plan p, const void* f_dev, void* fhat_dev);
+
+/* mul!(f, adjoint(p), fHat): src/implementation.jl:176-193 (interface.jl:178). */
+int nfftm_adjoint(nfftm_plan p, const void* fhat_dev, void* f_dev);
+
+/* deconvolve!(p, f, g): src/deconvolution.jl:2-45, including the
+ * fill!(tmpVec, 0) of src/implementation.jl:159.  g: prod(Ntilde) complex. */
+int nfftm_deconvolve(nfftm_plan p, const void* f_dev, void* g_dev);
+
+/* deconvolve_transpose!(p, g, f): src/deconvolution.jl:49-92. */
+int nfftm_deconvolve_transpose(nfftm_plan p, const void* g_dev, void* f_dev);
+
+/* convolve!(p, g, fHat): src/convolution.jl:20-45 (interpolation). */
+int nfftm_convolve(nfftm_plan p, const void* g_dev, void* fhat_dev);
+
+/* convolve_transpose!(p, fHat, g): src/convolution.jl:115-140 (spreading);
+ * g is overwritten. */
+int nfftm_convolve_transpose(nfftm_plan p, const void* fhat_dev, void* g_dev);
+
+/* Oversampled FFT stage on a caller grid, in place, unnormalised:
+ * p.forwardFFT * g (sign -1) or p.backwardFFT * g (sign +1),
+ * src/implementation.jl:87-88, 161, 182.  direction: -1 or +1. */
+int nfftm_fft(nfftm_plan p, void* g_dev, int direction);
+
+/* Plan geometry: Ntilde (D entries), M, and the effective sigma
+ * (src/precomputation.jl:25-29). */
+int nfftm_sizes(nfftm_plan p, int64_t* Ntilde, int64_t* M, double* sigma);
+
+/* The plan's internal grid (p.tmpVec) as a device pointer. */
+int nfftm_grid(nfftm_plan p, void** g_dev);
+
+/* TimingStats fields (AbstractNFFTs/src/misc.jl:22-30), seconds:
+ * {pre, conv, fft, deconv, conv_adjoint, fft_adjoint, deconv_adjoint}. */
+int nfftm_timing(nfftm_plan p, double t[7]);
+
+/* Switch per-stage timing on (1) or off (0) for later transform calls; the
+ * per-call `timing=` keyword of mul! (src/implementation.jl:155-193). */
+int nfftm_set_timing(nfftm_plan p, int on);
+
+/* Change the HIP stream the plan enqueues on. */
+int nfftm_set_stream(nfftm_plan p, void* hip_stream);
+
+/* copy(p): src/implementation.jl:45-66.  The copy has its own grid. */
+int nfftm_plan_copy(nfftm_plan src, nfftm_plan* dst);
+
+/* finalizer (Wrappers/FINUFFT.jl:52-56 pattern). */
+void nfftm_plan_destroy(nfftm_plan p);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* nfftm_last_error(void);
+
+/* Library version string. */
+const char* nfftm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NFFTM_H */

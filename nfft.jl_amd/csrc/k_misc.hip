@@ -1,0 +1,240 @@
+// k_misc.hip -- deconvolution (apodisation + zero padding + fftshift
+// placement), grid clearing, and the node-parallel any-D gridding kernels.
+#include "launch.hpp"
+
+namespace nfftm {
+
+// Forward deconvolution fused with the grid clear: every grid point of the
+// oversampled grid is written once, either with f_n * prod_d 1/psi_hat_d(n_d)
+// (image index n inside the fftshifted corners) or with zero.  This is
+// fill!(tmpVec, 0) (src/implementation.jl:159) followed by deconvolve!
+// (src/deconvolution.jl:22-45): image index i_d < Na_d goes to grid index
+// i_d - Na_d + Nt_d, the rest to i_d - Na_d.  The factor product is FP64 in
+// both precisions.
+template <typename R>
+__global__ __launch_bounds__(256) void k_deconv(DArgs a, const typename cpx<R>::T* __restrict__ f,
+                                                typename cpx<R>::T* __restrict__ g) {
+  using C = typename cpx<R>::T;
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int nrows = (int)(a.Q / a.Nt[0]);
+  const int Nt0 = a.Nt[0], Na0 = a.Na[0], N0 = a.N[0];
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+    // outer dims of this row (uniform over the block)
+    bool inside = true;
+    long long fo = 0;
+    double facr = 1.0;
+    int r = row;
+    for (int d = 1; d < a.D; ++d) {
+      const int gd = r % a.Nt[d];
+      r /= a.Nt[d];
+      int id;
+      if (gd < a.N[d] - a.Na[d]) id = gd + a.Na[d];
+      else if (gd >= a.Nt[d] - a.Na[d]) id = gd - (a.Nt[d] - a.Na[d]);
+      else { inside = false; id = 0; }
+      fo += (long long)id * a.nstr[d];
+      facr *= a.fac[d][id];
+    }
+    if (x >= Nt0) continue;
+    C out;
+    out.x = 0;
+    out.y = 0;
+    int i0 = -1;
+    if (x < N0 - Na0) i0 = x + Na0;
+    else if (x >= Nt0 - Na0) i0 = x - (Nt0 - Na0);
+    if (inside && i0 >= 0) {
+      const C v = f[fo + i0];
+      const double s = a.fac[0][i0] * facr;
+      out.x = (R)((double)v.x * s);
+      out.y = (R)((double)v.y * s);
+    }
+    g[(long long)row * Nt0 + x] = out;
+  }
+}
+
+// deconvolve_transpose! (src/deconvolution.jl:69-92): f_n = g[(n mod Nt)] *
+// prod_d 1/psi_hat_d(n_d).
+template <typename R>
+__global__ __launch_bounds__(256) void k_deconv_t(DArgs a, const typename cpx<R>::T* __restrict__ g,
+                                                  typename cpx<R>::T* __restrict__ f) {
+  using C = typename cpx<R>::T;
+  const int i0 = blockIdx.x * 256 + threadIdx.x;
+  const int N0 = a.N[0], Na0 = a.Na[0];
+  const int nrows = (int)(a.P / N0);
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+    long long go = 0;
+    double facr = 1.0;
+    int r = row;
+    for (int d = 1; d < a.D; ++d) {
+      const int id = r % a.N[d];
+      r /= a.N[d];
+      const int gd = id < a.Na[d] ? id - a.Na[d] + a.Nt[d] : id - a.Na[d];
+      go += (long long)gd * a.gstr[d];
+      facr *= a.fac[d][id];
+    }
+    if (i0 >= N0) continue;
+    const int gx = i0 < Na0 ? i0 - Na0 + a.Nt[0] : i0 - Na0;
+    const C v = g[go + gx];
+    const double s = a.fac[0][i0] * facr;
+    C out;
+    out.x = (R)((double)v.x * s);
+    out.y = (R)((double)v.y * s);
+    f[(long long)row * N0 + i0] = out;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill0(uint4* __restrict__ p, long long n16) {
+  const uint4 z = {0u, 0u, 0u, 0u};
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n16; i += (long long)gridDim.x * 256)
+    p[i] = z;
+}
+
+__global__ void k_fill0_tail(unsigned char* __restrict__ p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Node-parallel gridding for any D (the non-blocked path of
+// src/convolution.jl:56-99 and :152-200): one thread per node, footprint
+// walked with an odometer, spreading by global atomics.
+template <typename R, int W>
+__global__ __launch_bounds__(256) void k_spread_generic(KArgs a, const typename cpx<R>::T* __restrict__ fhat,
+                                                        typename cpx<R>::T* __restrict__ grid) {
+  using C = typename cpx<R>::T;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= a.M) return;
+  R w[MAXD][W];
+  long long base[MAXD];
+  for (int d = 0; d < a.D; ++d) {
+    const int cw = node_weights<R, W>(a, d, a.ks[(size_t)d * a.M + j], w[d]);
+    base[d] = (long long)cw - a.m + 1;
+  }
+  const C f = fhat[j];
+  R* g = (R*)grid;
+  int l[MAXD];
+  for (int d = 0; d < MAXD; ++d) l[d] = 0;
+  while (true) {
+    R pw = 1;
+    long long gi = 0;
+    for (int d = a.D - 1; d >= 0; --d) {
+      pw = pw * w[d][l[d]];
+      gi += wrapidx(base[d] + l[d], a.Nt[d]) * a.gstr[d];
+    }
+    gbl_add(&g[2 * gi], f.x * pw);
+    gbl_add(&g[2 * gi + 1], f.y * pw);
+    int d = 0;
+    for (; d < a.D; ++d) {
+      if (++l[d] < W) break;
+      l[d] = 0;
+    }
+    if (d == a.D) break;
+  }
+}
+
+template <typename R, int W>
+__global__ __launch_bounds__(256) void k_interp_generic(KArgs a, const typename cpx<R>::T* __restrict__ grid,
+                                                        typename cpx<R>::T* __restrict__ fhat) {
+  using C = typename cpx<R>::T;
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= a.M) return;
+  R w[MAXD][W];
+  long long base[MAXD];
+  for (int d = 0; d < a.D; ++d) {
+    const int cw = node_weights<R, W>(a, d, a.ks[(size_t)d * a.M + j], w[d]);
+    base[d] = (long long)cw - a.m + 1;
+  }
+  int l[MAXD];
+  for (int d = 0; d < MAXD; ++d) l[d] = 0;
+  R sr = 0, si = 0;
+  while (true) {
+    R pw = 1;
+    long long gi = 0;
+    for (int d = a.D - 1; d >= 0; --d) {
+      pw = pw * w[d][l[d]];
+      gi += wrapidx(base[d] + l[d], a.Nt[d]) * a.gstr[d];
+    }
+    const C v = grid[gi];
+    sr = fma(pw, v.x, sr);
+    si = fma(pw, v.y, si);
+    int d = 0;
+    for (; d < a.D; ++d) {
+      if (++l[d] < W) break;
+      l[d] = 0;
+    }
+    if (d == a.D) break;
+  }
+  fhat[j] = C{sr, si};
+}
+
+template <typename R>
+static hipError_t generic_w(bool spread, int W, const KArgs& a, hipStream_t s, const void* in, void* out) {
+  using C = typename cpx<R>::T;
+  const dim3 grid((a.M + 255) / 256), blk(256);
+#define NFFTM_GEN(WW)                                                                        \
+  case WW:                                                                                   \
+    if (spread)                                                                              \
+      hipLaunchKernelGGL((k_spread_generic<R, WW>), grid, blk, 0, s, a, (const C*)in, (C*)out); \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_interp_generic<R, WW>), grid, blk, 0, s, a, (const C*)in, (C*)out); \
+    break;
+  switch (W) {
+    NFFTM_GEN(2) NFFTM_GEN(4) NFFTM_GEN(6) NFFTM_GEN(8) NFFTM_GEN(10) NFFTM_GEN(12) NFFTM_GEN(14)
+    NFFTM_GEN(16)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef NFFTM_GEN
+  return hipGetLastError();
+}
+
+hipError_t launch_generic(bool spread, bool dbl, int W, const KArgs& a, hipStream_t s, const void* in,
+                          void* out) {
+  if (a.M == 0) return hipSuccess;
+  return dbl ? generic_w<double>(spread, W, a, s, in, out) : generic_w<float>(spread, W, a, s, in, out);
+}
+
+hipError_t launch_deconv(bool dbl, const DArgs& a, hipStream_t s, const void* f, void* g) {
+  const long long rows = a.Q / a.Nt[0];
+  const dim3 grid((a.Nt[0] + 255) / 256, (unsigned)std::min<long long>(rows, 65535)), blk(256);
+  if (dbl)
+    hipLaunchKernelGGL(k_deconv<double>, grid, blk, 0, s, a, (const double2*)f, (double2*)g);
+  else
+    hipLaunchKernelGGL(k_deconv<float>, grid, blk, 0, s, a, (const float2*)f, (float2*)g);
+  return hipGetLastError();
+}
+
+hipError_t launch_deconv_t(bool dbl, const DArgs& a, hipStream_t s, const void* g, void* f) {
+  const long long rows = a.P / a.N[0];
+  const dim3 grid((a.N[0] + 255) / 256, (unsigned)std::min<long long>(rows, 65535)), blk(256);
+  if (dbl)
+    hipLaunchKernelGGL(k_deconv_t<double>, grid, blk, 0, s, a, (const double2*)g, (double2*)f);
+  else
+    hipLaunchKernelGGL(k_deconv_t<float>, grid, blk, 0, s, a, (const float2*)g, (float2*)f);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s) {
+  const long long n16 = (long long)(bytes / 16);
+  if (n16 > 0) {
+    const long long blocks = std::min<long long>((n16 + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill0, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)p, n16);
+  }
+  const long long tail = (long long)(bytes - (size_t)n16 * 16);
+  if (tail > 0)
+    hipLaunchKernelGGL(k_fill0_tail, dim3(1), dim3(64), 0, s, (unsigned char*)p + n16 * 16, tail);
+  return hipGetLastError();
+}
+
+size_t tile_lds_bytes(int D, bool dbl, int W, int tileElems) {
+  const int NB = D == 1 ? Batch<1>::NB : D == 2 ? Batch<2>::NB : Batch<3>::NB;
+  return tile_layout_bytes(D, dbl ? sizeof(double) : sizeof(float), W, tileElems, NB);
+}
+
+hipError_t launch_tile(int D, bool spread, bool dbl, int W, int NT, const KArgs& a, size_t lds, hipStream_t s,
+                       const void* in, void* out) {
+  if (a.nitems == 0) return hipSuccess;
+  if (D == 1) return launch_tile_d1(spread, dbl, W, NT, a, lds, s, in, out);
+  if (D == 2) return launch_tile_d2(spread, dbl, W, NT, a, lds, s, in, out);
+  return launch_tile_d3(spread, dbl, W, NT, a, lds, s, in, out);
+}
+
+}  // namespace nfftm

@@ -1,0 +1,12 @@
+// Tile kernels for D = 1 (see kernels.hpp).
+#include "launch.hpp"
+#define TILE_D 1
+#include "tile_launch.inc"
+
+namespace nfftm {
+hipError_t launch_tile_d1(bool spread, bool dbl, int W, int NT, const KArgs& a, size_t lds,
+                          hipStream_t s, const void* in, void* out) {
+  return dbl ? tile_w<double>(spread, W, NT, a, lds, s, in, out)
+             : tile_w<float>(spread, W, NT, a, lds, s, in, out);
+}
+}  // namespace nfftm

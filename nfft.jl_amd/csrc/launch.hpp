@@ -1,0 +1,39 @@
+// launch.hpp -- host-side launchers of the device kernels (one translation
+// unit per tile dimensionality so they compile in parallel).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "kernels.hpp"
+
+namespace nfftm {
+
+// Arguments of the deconvolution kernels (src/deconvolution.jl).
+struct DArgs {
+  int D;
+  int N[MAXD], Nt[MAXD], Na[MAXD];
+  long long nstr[MAXD], gstr[MAXD];
+  const double* fac[MAXD];  // 1/psi_hat per dim, FP64, N[d] entries
+  long long P, Q;
+};
+
+// LDS bytes of a tile kernel.
+size_t tile_lds_bytes(int D, bool dbl, int W, int tileElems);
+
+// Tile kernels (D = 1, 2, 3).  grid = 8 * ceil(nitems / 8) workgroups of NT.
+hipError_t launch_tile(int D, bool spread, bool dbl, int W, int NT, const KArgs& a, size_t lds,
+                       hipStream_t s, const void* in, void* out);
+hipError_t launch_tile_d1(bool spread, bool dbl, int W, int NT, const KArgs& a, size_t lds,
+                          hipStream_t s, const void* in, void* out);
+hipError_t launch_tile_d2(bool spread, bool dbl, int W, int NT, const KArgs& a, size_t lds,
+                          hipStream_t s, const void* in, void* out);
+hipError_t launch_tile_d3(bool spread, bool dbl, int W, int NT, const KArgs& a, size_t lds,
+                          hipStream_t s, const void* in, void* out);
+
+// Node-parallel kernels for any D (no tiles; global atomics for spreading).
+hipError_t launch_generic(bool spread, bool dbl, int W, const KArgs& a, hipStream_t s,
+                          const void* in, void* out);
+
+hipError_t launch_deconv(bool dbl, const DArgs& a, hipStream_t s, const void* f, void* g);
+hipError_t launch_deconv_t(bool dbl, const DArgs& a, hipStream_t s, const void* g, void* f);
+hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s);
+
+}  // namespace nfftm

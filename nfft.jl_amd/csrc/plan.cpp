@@ -1,0 +1,796 @@
+// plan.cpp -- host side of libnfftm: plan construction (parameters, window
+// tables, deconvolution factors, FP64 node binning into tiles), the rocFFT
+// stage, the transform orchestration, and the C ABI of include/nfftm.h.
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nfftm.h"
+#include "launch.hpp"
+
+using namespace nfftm;
+typedef int64_t i64;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(e_ == hipErrorOutOfMemory ? NFFTM_ERR_OOM : NFFTM_ERR_HIP, "%s: %s", #x, \
+                  hipGetErrorString(e_));                                                 \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Kaiser-Bessel window and its Fourier transform (src/windowFunctions.jl:21-39)
+// ---------------------------------------------------------------------------
+double i0_series(double x) {
+  // I0(x) = sum_k ((x/2)^2)^k / (k!)^2; positive terms, relative error ~ulp.
+  const double y = 0.25 * x * x;
+  double t = 1.0, s = 1.0;
+  for (int k = 1; k < 1000 && t > 1e-17 * s; ++k) {
+    t = t * y / ((double)k * k);
+    s += t;
+  }
+  return s;
+}
+
+// psi(u) with u in grid units (the reference's window(k, Ntilde, m, sigma) at
+// k = u / Ntilde; the lookup table uses Ntilde = 1).
+double kb_psi(double u, int m, double b) {
+  const double au = std::fabs(u);
+  if (au < m) {
+    const double arg = std::sqrt((double)m * m - u * u);
+    return std::sinh(b * arg) / (arg * M_PI);
+  }
+  if (au > m) return 0.0;
+  return b / M_PI;
+}
+
+double kb_psi_hat(double n, double Nt, int m, double b) {
+  const double t = 2.0 * M_PI * n / Nt;
+  return i0_series(m * std::sqrt(b * b - t * t));
+}
+
+// LUTSize = 2^K * m with K from NFFT3's table (src/precomputation.jl:14-17).
+i64 lut_size(int m) {
+  static const int K[9] = {1, 3, 7, 9, 14, 17, 20, 23, 24};
+  return ((i64)1 << K[std::min(m, 8)]) * m;
+}
+
+// Least-squares polynomial fit of each stencil column
+// (src/precomputation.jl:302-320): sample points s_q = -1/2 + q/(2*deg-1),
+// q = 0..2*deg-1, deg = 2m+1 monomials; column l (0-based) fits
+// psi(m - (l + 1/2) + x).  Solved by Householder QR of the Vandermonde matrix.
+std::vector<double> poly_fit(int m, double b) {
+  const int deg = 2 * m + 1, W = 2 * m, ns = 2 * deg;
+  std::vector<double> x(ns);
+  for (int q = 0; q < ns; ++q) x[q] = (double)(2 * q - (ns - 1)) / (double)(2 * (ns - 1));
+  // V (ns x deg, column-major) and its QR factors, shared by all columns
+  std::vector<double> V((size_t)ns * deg);
+  for (int q = 0; q < ns; ++q) {
+    double p = 1.0;
+    for (int r = 0; r < deg; ++r) {
+      V[q + (size_t)r * ns] = p;
+      p *= x[q];
+    }
+  }
+  std::vector<std::vector<double>> house(deg);
+  std::vector<double> beta(deg, 0.0);
+  for (int c = 0; c < deg; ++c) {
+    double nrm2 = 0;
+    for (int q = c; q < ns; ++q) nrm2 += V[q + (size_t)c * ns] * V[q + (size_t)c * ns];
+    const double nrm = std::sqrt(nrm2);
+    const double a0 = V[c + (size_t)c * ns];
+    const double alpha = a0 >= 0 ? -nrm : nrm;
+    std::vector<double> v(ns, 0.0);
+    v[c] = a0 - alpha;
+    for (int q = c + 1; q < ns; ++q) v[q] = V[q + (size_t)c * ns];
+    double vv = 0;
+    for (int q = c; q < ns; ++q) vv += v[q] * v[q];
+    beta[c] = vv > 0 ? 2.0 / vv : 0.0;
+    for (int cc = c; cc < deg; ++cc) {
+      double s = 0;
+      for (int q = c; q < ns; ++q) s += v[q] * V[q + (size_t)cc * ns];
+      s *= beta[c];
+      for (int q = c; q < ns; ++q) V[q + (size_t)cc * ns] -= s * v[q];
+    }
+    house[c] = std::move(v);
+  }
+  std::vector<double> P((size_t)deg * W);
+  for (int l = 0; l < W; ++l) {
+    std::vector<double> y(ns);
+    const double c0 = (double)m - ((double)l + 0.5);
+    for (int q = 0; q < ns; ++q) y[q] = kb_psi(c0 + x[q], m, b);
+    for (int c = 0; c < deg; ++c) {
+      double s = 0;
+      for (int q = c; q < ns; ++q) s += house[c][q] * y[q];
+      s *= beta[c];
+      for (int q = c; q < ns; ++q) y[q] -= s * house[c][q];
+    }
+    for (int r = deg - 1; r >= 0; --r) {
+      double s = y[r];
+      for (int cc = r + 1; cc < deg; ++cc) s -= V[r + (size_t)cc * ns] * P[cc + (size_t)l * deg];
+      P[r + (size_t)l * deg] = s / V[r + (size_t)r * ns];
+    }
+  }
+  return P;
+}
+
+// ---------------------------------------------------------------------------
+// rocFFT stage: in place, unnormalised, all D dimensions, Julia order
+// (dimension 1 = rocFFT lengths[0]).  D <= 3 is one plan; D > 3 is a 3-D plan
+// batched over the outer dimensions followed by strided 1-D plans.
+// ---------------------------------------------------------------------------
+std::once_flag g_rocfft_once;
+
+struct FFTPass {
+  rocfft_plan plan = nullptr;
+  i64 nexec = 1, exec_stride = 0;  // executions at pointer offsets (elements)
+};
+
+struct FFTStage {
+  std::vector<FFTPass> fwd, bwd;
+  rocfft_execution_info info = nullptr;
+  void* work = nullptr;
+  size_t worksize = 0;
+
+  ~FFTStage() { release(); }
+  void release() {
+    for (auto* v : {&fwd, &bwd})
+      for (auto& p : *v)
+        if (p.plan) rocfft_plan_destroy(p.plan);
+    fwd.clear();
+    bwd.clear();
+    if (info) rocfft_execution_info_destroy(info);
+    info = nullptr;
+    if (work) (void)hipFree(work);
+    work = nullptr;
+  }
+
+  int make_pass(std::vector<FFTPass>& out, bool forward, bool dbl, int dim, const size_t* len,
+                const size_t* stride, size_t dist, size_t batch, i64 nexec, i64 exec_stride) {
+    rocfft_plan_description desc = nullptr;
+    if (rocfft_plan_description_create(&desc) != rocfft_status_success)
+      return fail(NFFTM_ERR_FFT, "rocfft_plan_description_create failed");
+    rocfft_status st = rocfft_plan_description_set_data_layout(
+        desc, rocfft_array_type_complex_interleaved, rocfft_array_type_complex_interleaved, nullptr,
+        nullptr, dim, stride, dist, dim, stride, dist);
+    FFTPass p;
+    if (st == rocfft_status_success)
+      st = rocfft_plan_create(&p.plan, rocfft_placement_inplace,
+                              forward ? rocfft_transform_type_complex_forward
+                                      : rocfft_transform_type_complex_inverse,
+                              dbl ? rocfft_precision_double : rocfft_precision_single, dim, len, batch,
+                              desc);
+    rocfft_plan_description_destroy(desc);
+    if (st != rocfft_status_success) return fail(NFFTM_ERR_FFT, "rocfft_plan_create failed (%d)", (int)st);
+    size_t ws = 0;
+    rocfft_plan_get_work_buffer_size(p.plan, &ws);
+    worksize = std::max(worksize, ws);
+    p.nexec = nexec;
+    p.exec_stride = exec_stride;
+    out.push_back(p);
+    return NFFTM_OK;
+  }
+
+  int build(int D, const i64* Nt, bool dbl) {
+    std::call_once(g_rocfft_once, [] { rocfft_setup(); });
+    for (int dir = 0; dir < 2; ++dir) {
+      auto& v = dir == 0 ? fwd : bwd;
+      const bool f = dir == 0;
+      const int d3 = std::min(D, 3);
+      size_t len[3], str[3];
+      size_t s = 1;
+      for (int d = 0; d < d3; ++d) {
+        len[d] = (size_t)Nt[d];
+        str[d] = s;
+        s *= (size_t)Nt[d];
+      }
+      size_t outer = 1;
+      for (int d = 3; d < D; ++d) outer *= (size_t)Nt[d];
+      int rc = make_pass(v, f, dbl, d3, len, str, s, outer, 1, 0);
+      if (rc) return rc;
+      size_t inner = s;
+      for (int d = 3; d < D; ++d) {
+        size_t after = 1;
+        for (int e = d + 1; e < D; ++e) after *= (size_t)Nt[e];
+        size_t l1[1] = {(size_t)Nt[d]}, s1[1] = {inner};
+        rc = make_pass(v, f, dbl, 1, l1, s1, 1, inner, (i64)after, (i64)(inner * Nt[d]));
+        if (rc) return rc;
+        inner *= (size_t)Nt[d];
+      }
+    }
+    if (rocfft_execution_info_create(&info) != rocfft_status_success)
+      return fail(NFFTM_ERR_FFT, "rocfft_execution_info_create failed");
+    if (worksize) {
+      if (hipMalloc(&work, worksize) != hipSuccess) return fail(NFFTM_ERR_OOM, "rocFFT work buffer");
+      rocfft_execution_info_set_work_buffer(info, work, worksize);
+    }
+    return NFFTM_OK;
+  }
+
+  int exec(bool forward, void* g, size_t elem, hipStream_t s) {
+    rocfft_execution_info_set_stream(info, s);
+    for (auto& p : forward ? fwd : bwd)
+      for (i64 e = 0; e < p.nexec; ++e) {
+        void* buf[1] = {(char*)g + (size_t)(e * p.exec_stride) * elem};
+        if (rocfft_execute(p.plan, buf, nullptr, info) != rocfft_status_success)
+          return fail(NFFTM_ERR_FFT, "rocfft_execute failed");
+      }
+    return NFFTM_OK;
+  }
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    reset();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+};
+
+}  // namespace
+
+// ===========================================================================
+// The plan
+// ===========================================================================
+struct nfftm_plan_s {
+  // parameters (NFFTParams, src/implementation.jl:3-14)
+  int prec = NFFTM_F32, D = 1, m = 4, W = 8, window = 0, precompute = NFFTM_LINEAR, flags = 0;
+  double sigma_in = 2.0, sigma = 2.0, b = 0.0, wscale = 1.0;
+  i64 N[MAXD] = {0}, Nt[MAXD] = {0}, P = 1, Q = 1, M = 0, LUTSize = 0;
+  bool dbl() const { return prec == NFFTM_F64; }
+  size_t celem() const { return dbl() ? 16 : 8; }
+  bool tiled = true;
+  hipStream_t stream = nullptr;
+
+  // tables (host FP64 masters; device copies in the data precision)
+  std::vector<double> lut, poly;
+  std::vector<std::vector<double>> deconv;
+  DevBuf<char> d_lut, d_poly, d_grid;
+  DevBuf<double> d_deconv, d_ks;
+  DevBuf<int> d_perm;
+  DevBuf<int4> d_items;
+  int nitems = 0;
+
+  // tiling
+  int T[3] = {1, 1, 1}, ntiles[3] = {1, 1, 1}, Pd[3] = {1, 1, 1}, Sx = 1, Sxy = 1, tileElems = 1, NT = 256;
+  size_t lds = 0;
+
+  std::unique_ptr<FFTStage> fft;
+  double timing[7] = {0};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<double> k;  // caller's nodes (D x M), kept for copy()
+
+  ~nfftm_plan_s() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+
+  KArgs kargs() const {
+    KArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.D = D;
+    a.m = m;
+    a.W = W;
+    a.mode = precompute == NFFTM_LINEAR ? WIN_LINEAR : precompute == NFFTM_FULL ? WIN_EXACT : WIN_POLY;
+    a.M = (int)M;
+    i64 s = 1;
+    for (int d = 0; d < D; ++d) {
+      a.Nt[d] = (int)Nt[d];
+      a.Ntd[d] = (double)Nt[d];
+      a.gstr[d] = s;
+      s *= Nt[d];
+    }
+    for (int d = 0; d < 3; ++d) {
+      a.T[d] = T[d];
+      a.ntiles[d] = ntiles[d];
+      a.Pd[d] = Pd[d];
+    }
+    a.Sx = Sx;
+    a.Sxy = Sxy;
+    a.tileElems = tileElems;
+    a.padCells = Pd[0] * Pd[1] * Pd[2];
+    a.divP0 = make_fastdiv((uint32_t)Pd[0]);
+    a.divP1 = make_fastdiv((uint32_t)Pd[1]);
+    a.divNtx = make_fastdiv((uint32_t)std::max<i64>(1, Nt[0]));
+    a.lut = d_lut.p;
+    a.scale = LUTSize / m;
+    a.poly = d_poly.p;
+    a.b = b;
+    a.wscale = wscale;
+    a.ks = d_ks.p;
+    a.perm = d_perm.p;
+    a.items = d_items.p;
+    a.nitems = nitems;
+    return a;
+  }
+
+  DArgs dargs() const {
+    DArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.D = D;
+    i64 ns = 1, gs = 1, off = 0;
+    for (int d = 0; d < D; ++d) {
+      a.N[d] = (int)N[d];
+      a.Nt[d] = (int)Nt[d];
+      a.Na[d] = (int)(N[d] / 2);
+      a.nstr[d] = ns;
+      a.gstr[d] = gs;
+      ns *= N[d];
+      gs *= Nt[d];
+      a.fac[d] = d_deconv.p + off;
+      off += N[d];
+    }
+    a.P = P;
+    a.Q = Q;
+    return a;
+  }
+};
+
+namespace {
+
+// Tile geometry of the gridding kernels (D <= 3).  Sx is the padded row
+// extent rounded to 8 (mod 16) when that costs at most one element, which
+// keeps the 8x8 lane footprint of a 3-D stencil on distinct LDS banks.
+void choose_tiles(nfftm_plan_s& p) {
+  const int D = p.D, W = p.W;
+  const bool dbl = p.dbl();
+  const size_t budget = 158 * 1024;
+  auto fit = [&](int t0, int t1, int t2) {
+    int T[3] = {t0, t1, t2}, Pd[3] = {1, 1, 1};
+    for (int d = 0; d < D; ++d) {
+      T[d] = (int)std::min<i64>(T[d], p.Nt[d]);
+      Pd[d] = T[d] + W - 1;
+    }
+    int Sx = Pd[0];
+    if (D >= 2) {
+      const int r = Pd[0] % 16;
+      if (r == 7) Sx = Pd[0] + 1;
+    }
+    const int Sxy = Sx * (D >= 2 ? Pd[1] : 1);
+    const int te = Sxy * (D == 3 ? Pd[2] : 1);
+    const size_t bytes = tile_lds_bytes(D, dbl, W, te);
+    for (int d = 0; d < 3; ++d) {
+      p.T[d] = D > d ? T[d] : 1;
+      p.Pd[d] = D > d ? Pd[d] : 1;
+      p.ntiles[d] = D > d ? (int)((p.Nt[d] + T[d] - 1) / T[d]) : 1;
+    }
+    p.Sx = Sx;
+    p.Sxy = Sxy;
+    p.tileElems = te;
+    p.lds = bytes;
+    return bytes <= budget;
+  };
+  if (D == 3) {
+    p.NT = 512;
+    for (int t : {16, 12, 10, 8, 6, 4})
+      if (fit(t, t, t)) return;
+  } else if (D == 2) {
+    p.NT = 256;
+    int t = 64;
+    while (t > 16 && (double)p.Nt[0] * p.Nt[1] / ((double)t * t) < 256) t /= 2;
+    while (!fit(t, t, 1) && t > 4) t /= 2;
+  } else {
+    p.NT = 256;
+    int t = dbl ? 2048 : 4096;
+    while (t > 256 && (double)p.Nt[0] / t < 256) t /= 2;
+    while (!fit(t, 1, 1) && t > 64) t /= 2;
+  }
+}
+
+// Window normalisation by an exact power of two so the FP32 weights stay near
+// 1 (the unnormalised psi(0) reaches 1e15 at m = 8); the deconvolution factors
+// carry the inverse power, so the transform is unchanged.
+double window_scale(int m, double b) {
+  const double p0 = kb_psi(0.0, m, b);
+  return std::ldexp(1.0, -(int)std::lround(std::log2(p0)));
+}
+
+template <typename R>
+void upload_table(DevBuf<char>& buf, const std::vector<double>& v, double scale, hipError_t& e) {
+  std::vector<R> h(v.size());
+  for (size_t i = 0; i < v.size(); ++i) h[i] = (R)(v[i] * scale);
+  e = buf.alloc(h.size() * sizeof(R));
+  if (e == hipSuccess && !h.empty()) e = hipMemcpy(buf.p, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice);
+}
+
+// checkNodes (src/utils.jl:46-55): |k| <= 1/2, NaN rejected.
+int check_nodes(const double* k, i64 n) {
+  for (i64 i = 0; i < n; ++i)
+    if (!(std::fabs(k[i]) <= 0.5))
+      return fail(NFFTM_ERR_ARG, "Nodes k need to be within the range [-1/2, 1/2] but k[%lld] = %g",
+                  (long long)i, k[i]);
+  return NFFTM_OK;
+}
+
+// Node-dependent state (nodes!, src/implementation.jl:108-141): shift into
+// [0, 1) (src/utils.jl:32-44), bin by tile (src/precomputation.jl:487-520 at
+// tile granularity), split each tile's nodes into work items.
+int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
+  const int D = p.D;
+  if (M < 0 || M > 0x7fffffff) return fail(NFFTM_ERR_UNSUPPORTED, "M = %lld out of range", (long long)M);
+  if (int rc = check_nodes(k, (i64)D * M)) return rc;
+  p.M = M;
+  p.k.assign(k, k + (size_t)D * M);
+  std::vector<double> ks((size_t)D * M);
+  std::vector<int> perm;
+  std::vector<int4> items;
+  auto shifted = [&](i64 j, int d) {
+    double x = k[d + (size_t)D * j];
+    if (x < 0.0) x += 1.0;
+    if (x == 1.0) x -= 2.220446049250313e-16;  // eps(Float64)
+    return x;
+  };
+  if (!p.tiled) {
+    for (i64 j = 0; j < M; ++j)
+      for (int d = 0; d < D; ++d) ks[(size_t)d * M + j] = shifted(j, d);
+    perm.resize(M);
+    for (i64 j = 0; j < M; ++j) perm[j] = (int)j;
+  } else {
+    const i64 ntile = (i64)p.ntiles[0] * p.ntiles[1] * p.ntiles[2];
+    std::vector<int> tile_of(M);
+    std::vector<i64> cnt(ntile + 1, 0);
+    for (i64 j = 0; j < M; ++j) {
+      i64 t = 0, str = 1;
+      for (int d = 0; d < D; ++d) {
+        const double kscale = shifted(j, d) * (double)p.Nt[d];
+        i64 c = (i64)kscale;
+        if (c >= p.Nt[d]) c -= p.Nt[d];
+        t += (c / p.T[d]) * str;
+        str *= p.ntiles[d];
+      }
+      tile_of[j] = (int)t;
+      cnt[t + 1]++;
+    }
+    for (i64 t = 0; t < ntile; ++t) cnt[t + 1] += cnt[t];
+    perm.resize(M);
+    {
+      std::vector<i64> pos(cnt.begin(), cnt.end() - 1);
+      for (i64 j = 0; j < M; ++j) perm[pos[tile_of[j]]++] = (int)j;
+    }
+    for (i64 s = 0; s < M; ++s)
+      for (int d = 0; d < D; ++d) ks[(size_t)d * M + s] = shifted(perm[s], d);
+    // work items: chunks of at most CH nodes of one tile
+    const i64 CH = D == 3 ? 1024 : D == 2 ? 2048 : 4096;
+    for (i64 t = 0; t < ntile; ++t) {
+      for (i64 b0 = cnt[t]; b0 < cnt[t + 1]; b0 += CH) {
+        const i64 e0 = std::min(cnt[t + 1], b0 + CH);
+        items.push_back(int4{(int)t, (int)b0, (int)e0, 0});
+      }
+    }
+  }
+  p.nitems = (int)items.size();
+  HIPCHK(p.d_ks.alloc(ks.size()));
+  HIPCHK(p.d_perm.alloc(perm.size()));
+  HIPCHK(p.d_items.alloc(items.size()));
+  if (!ks.empty()) HIPCHK(hipMemcpy(p.d_ks.p, ks.data(), ks.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (!perm.empty())
+    HIPCHK(hipMemcpy(p.d_perm.p, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!items.empty())
+    HIPCHK(hipMemcpy(p.d_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+  return NFFTM_OK;
+}
+
+int build_tables(nfftm_plan_s& p) {
+  const int m = p.m;
+  p.b = M_PI * (2.0 - 1.0 / p.sigma);
+  p.wscale = window_scale(m, p.b);
+  hipError_t e = hipSuccess;
+  if (p.precompute == NFFTM_LINEAR) {
+    // precomputeLinInterp (src/precomputation.jl:291-300): K+2 samples, step m/K
+    const i64 K = p.LUTSize;
+    p.lut.resize(K + 2);
+    const double step = (double)m / (double)K;
+    for (i64 l = 0; l < K + 2; ++l) p.lut[l] = kb_psi((double)l * step, m, p.b);
+    if (p.dbl()) upload_table<double>(p.d_lut, p.lut, p.wscale, e);
+    else upload_table<float>(p.d_lut, p.lut, p.wscale, e);
+  } else if (p.precompute == NFFTM_TENSOR || p.precompute == NFFTM_POLYNOMIAL) {
+    p.poly = poly_fit(m, p.b);
+    if (p.dbl()) upload_table<double>(p.d_poly, p.poly, p.wscale, e);
+    else upload_table<float>(p.d_poly, p.poly, p.wscale, e);
+  }
+  HIPCHK(e);
+  // deconvolution factors 1/psi_hat(n), n = i - floor(N/2)
+  // (src/precomputation.jl:345-358), FP64, times the inverse window scale
+  p.deconv.assign(p.D, {});
+  std::vector<double> flat;
+  for (int d = 0; d < p.D; ++d) {
+    p.deconv[d].resize(p.N[d]);
+    const i64 Na = p.N[d] / 2;
+    for (i64 i = 0; i < p.N[d]; ++i)
+      p.deconv[d][i] = 1.0 / kb_psi_hat((double)(i - Na), (double)p.Nt[d], m, p.b) / p.wscale;
+    flat.insert(flat.end(), p.deconv[d].begin(), p.deconv[d].end());
+  }
+  HIPCHK(p.d_deconv.alloc(flat.size()));
+  HIPCHK(hipMemcpy(p.d_deconv.p, flat.data(), flat.size() * sizeof(double), hipMemcpyHostToDevice));
+  return NFFTM_OK;
+}
+
+int run_conv(nfftm_plan_s& p, bool spread, const void* in, void* out) {
+  const KArgs a = p.kargs();
+  hipError_t e;
+  if (spread) HIPCHK(launch_fill0(out, (size_t)p.Q * p.celem(), p.stream));
+  if (p.tiled)
+    e = launch_tile(p.D, spread, p.dbl(), p.W, p.NT, a, p.lds, p.stream, in, out);
+  else
+    e = launch_generic(spread, p.dbl(), p.W, a, p.stream, in, out);
+  HIPCHK(e);
+  return NFFTM_OK;
+}
+
+int run_deconv(nfftm_plan_s& p, bool fwd, const void* in, void* out) {
+  const DArgs a = p.dargs();
+  if (fwd) HIPCHK(launch_deconv(p.dbl(), a, p.stream, in, out));
+  else HIPCHK(launch_deconv_t(p.dbl(), a, p.stream, in, out));
+  return NFFTM_OK;
+}
+
+struct Timer {
+  nfftm_plan_s& p;
+  bool on;
+  explicit Timer(nfftm_plan_s& pp) : p(pp), on(pp.flags & NFFTM_FLAG_TIMING) {}
+  int mark(int i) {
+    if (on) HIPCHK(hipEventRecord(p.ev[i], p.stream));
+    return NFFTM_OK;
+  }
+  int finish(double* t1, double* t2, double* t3) {
+    if (!on) return NFFTM_OK;
+    HIPCHK(hipEventSynchronize(p.ev[3]));
+    float a = 0, b = 0, c = 0;
+    HIPCHK(hipEventElapsedTime(&a, p.ev[0], p.ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, p.ev[1], p.ev[2]));
+    HIPCHK(hipEventElapsedTime(&c, p.ev[2], p.ev[3]));
+    *t1 = a * 1e-3;
+    *t2 = b * 1e-3;
+    *t3 = c * 1e-3;
+    return NFFTM_OK;
+  }
+};
+
+int create_impl(nfftm_plan_s& p, const double* k) {
+  // initParams (src/precomputation.jl:3-56)
+  p.W = 2 * p.m;
+  p.LUTSize = lut_size(p.m);
+  p.P = p.Q = 1;
+  for (int d = 0; d < p.D; ++d) {
+    p.Nt[d] = ((i64)std::ceil(p.sigma_in * (double)p.N[d]) / 2) * 2;
+    p.P *= p.N[d];
+    p.Q *= p.Nt[d];
+  }
+  p.sigma = (double)p.Nt[0] / (double)p.N[0];
+  for (int d = 0; d < p.D; ++d)
+    if (p.Nt[d] > 0x7fffffff)
+      return fail(NFFTM_ERR_UNSUPPORTED, "oversampled size %lld exceeds 2^31-1", (long long)p.Nt[d]);
+  p.tiled = (p.flags & NFFTM_FLAG_BLOCKING) && p.D <= 3;
+  if (p.tiled) choose_tiles(p);
+  if (int rc = build_tables(p)) return rc;
+  if (int rc = set_nodes(p, k, p.M)) return rc;
+  HIPCHK(p.d_grid.alloc((size_t)p.Q * p.celem()));
+  p.fft.reset(new FFTStage());
+  if (int rc = p.fft->build(p.D, p.Nt, p.dbl())) return rc;
+  for (auto& e : p.ev) HIPCHK(hipEventCreate(&e));
+  return NFFTM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* nfftm_last_error(void) { return g_err.c_str(); }
+const char* nfftm_version(void) { return "nfftm 0.1 (gfx950)"; }
+
+int nfftm_plan_create(nfftm_plan* out, int prec, int D, const int64_t* N, int64_t M, const double* k_host,
+                      int m, double sigma, int window, int precompute, int flags, void* hip_stream) {
+  g_err.clear();
+  if (!out) return fail(NFFTM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (prec != NFFTM_F32 && prec != NFFTM_F64 && prec != NFFTM_MIXED)
+    return fail(NFFTM_ERR_UNSUPPORTED, "precision %d not supported", prec);
+  if (D < 1 || D > MAXD) return fail(NFFTM_ERR_UNSUPPORTED, "D = %d not supported (1..%d)", D, MAXD);
+  if (!N) return fail(NFFTM_ERR_ARG, "N is NULL");
+  for (int d = 0; d < D; ++d)
+    if (N[d] < 1) return fail(NFFTM_ERR_ARG, "N[%d] = %lld must be positive", d, (long long)N[d]);
+  if (M > 0 && !k_host) return fail(NFFTM_ERR_ARG, "k is NULL");
+  if (window != NFFTM_KAISER_BESSEL) return fail(NFFTM_ERR_UNSUPPORTED, "Window %d not yet implemented!", window);
+  if (precompute < NFFTM_FULL || precompute > NFFTM_POLYNOMIAL)
+    return fail(NFFTM_ERR_UNSUPPORTED, "precompute = %d not supported", precompute);
+  if (m < 1 || m > 8) return fail(NFFTM_ERR_UNSUPPORTED, "m = %d not supported (1..8)", m);
+  if (!(sigma >= 1.0) || !std::isfinite(sigma))
+    return fail(NFFTM_ERR_ARG, "oversampling factor sigma = %g must be >= 1", sigma);
+  std::unique_ptr<nfftm_plan_s> p(new nfftm_plan_s());
+  p->prec = prec == NFFTM_F64 ? NFFTM_F64 : NFFTM_F32;
+  p->D = D;
+  p->m = m;
+  p->window = window;
+  p->precompute = precompute;
+  p->flags = flags;
+  p->sigma_in = sigma;
+  p->M = M;
+  p->stream = (hipStream_t)hip_stream;
+  for (int d = 0; d < D; ++d) p->N[d] = N[d];
+  // node validation happens before any device work (checkNodes in the ctor)
+  if (int rc = check_nodes(k_host, (i64)D * M)) return rc;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(NFFTM_ERR_HIP, "no HIP device available");
+  if (int rc = create_impl(*p, k_host)) return rc;
+  *out = p.release();
+  return NFFTM_OK;
+}
+
+int nfftm_nodes(nfftm_plan p, const double* k_host, int64_t M) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if (M > 0 && !k_host) return fail(NFFTM_ERR_ARG, "k is NULL");
+  if (int rc = check_nodes(k_host, (i64)p->D * M)) return rc;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return set_nodes(*p, k_host, M);
+}
+
+int nfftm_forward(nfftm_plan p, const void* f, void* fhat) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if ((!f && p->P) || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
+  Timer t(*p);
+  if (int rc = t.mark(0)) return rc;
+  if (int rc = run_deconv(*p, true, f, p->d_grid.p)) return rc;
+  if (int rc = t.mark(1)) return rc;
+  if (int rc = p->fft->exec(true, p->d_grid.p, p->celem(), p->stream)) return rc;
+  if (int rc = t.mark(2)) return rc;
+  if (p->M) {
+    if (int rc = run_conv(*p, false, p->d_grid.p, fhat)) return rc;
+  }
+  if (int rc = t.mark(3)) return rc;
+  return t.finish(&p->timing[3], &p->timing[2], &p->timing[1]);
+}
+
+int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if ((!f && p->P) || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
+  Timer t(*p);
+  if (int rc = t.mark(0)) return rc;
+  if (int rc = run_conv(*p, true, fhat, p->d_grid.p)) return rc;
+  if (int rc = t.mark(1)) return rc;
+  if (int rc = p->fft->exec(false, p->d_grid.p, p->celem(), p->stream)) return rc;
+  if (int rc = t.mark(2)) return rc;
+  if (int rc = run_deconv(*p, false, p->d_grid.p, f)) return rc;
+  if (int rc = t.mark(3)) return rc;
+  return t.finish(&p->timing[4], &p->timing[5], &p->timing[6]);
+}
+
+int nfftm_deconvolve(nfftm_plan p, const void* f, void* g) {
+  g_err.clear();
+  if (!p || !f || !g) return fail(NFFTM_ERR_ARG, "NULL argument");
+  return run_deconv(*p, true, f, g);
+}
+
+int nfftm_deconvolve_transpose(nfftm_plan p, const void* g, void* f) {
+  g_err.clear();
+  if (!p || !f || !g) return fail(NFFTM_ERR_ARG, "NULL argument");
+  return run_deconv(*p, false, g, f);
+}
+
+int nfftm_convolve(nfftm_plan p, const void* g, void* fhat) {
+  g_err.clear();
+  if (!p || !g || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL argument");
+  if (!p->M) return NFFTM_OK;
+  return run_conv(*p, false, g, fhat);
+}
+
+int nfftm_convolve_transpose(nfftm_plan p, const void* fhat, void* g) {
+  g_err.clear();
+  if (!p || !g || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL argument");
+  return run_conv(*p, true, fhat, g);
+}
+
+int nfftm_fft(nfftm_plan p, void* g, int direction) {
+  g_err.clear();
+  if (!p || !g) return fail(NFFTM_ERR_ARG, "NULL argument");
+  if (direction != -1 && direction != 1) return fail(NFFTM_ERR_ARG, "direction must be -1 or +1");
+  return p->fft->exec(direction == -1, g, p->celem(), p->stream);
+}
+
+int nfftm_sizes(nfftm_plan p, int64_t* Ntilde, int64_t* M, double* sigma) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if (Ntilde)
+    for (int d = 0; d < p->D; ++d) Ntilde[d] = p->Nt[d];
+  if (M) *M = p->M;
+  if (sigma) *sigma = p->sigma;
+  return NFFTM_OK;
+}
+
+int nfftm_grid(nfftm_plan p, void** g) {
+  g_err.clear();
+  if (!p || !g) return fail(NFFTM_ERR_ARG, "NULL argument");
+  *g = p->d_grid.p;
+  return NFFTM_OK;
+}
+
+int nfftm_timing(nfftm_plan p, double t[7]) {
+  g_err.clear();
+  if (!p || !t) return fail(NFFTM_ERR_ARG, "NULL argument");
+  for (int i = 0; i < 7; ++i) t[i] = p->timing[i];
+  return NFFTM_OK;
+}
+
+int nfftm_set_timing(nfftm_plan p, int on) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if (on) p->flags |= NFFTM_FLAG_TIMING;
+  else p->flags &= ~NFFTM_FLAG_TIMING;
+  return NFFTM_OK;
+}
+
+int nfftm_set_stream(nfftm_plan p, void* s) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  p->stream = (hipStream_t)s;
+  return NFFTM_OK;
+}
+
+int nfftm_plan_copy(nfftm_plan src, nfftm_plan* dst) {
+  g_err.clear();
+  if (!src || !dst) return fail(NFFTM_ERR_ARG, "NULL argument");
+  *dst = nullptr;
+  std::unique_ptr<nfftm_plan_s> p(new nfftm_plan_s());
+  p->prec = src->prec;
+  p->D = src->D;
+  p->m = src->m;
+  p->window = src->window;
+  p->precompute = src->precompute;
+  p->flags = src->flags;
+  p->sigma_in = src->sigma_in;
+  p->M = src->M;
+  p->stream = src->stream;
+  for (int d = 0; d < src->D; ++d) p->N[d] = src->N[d];
+  if (int rc = create_impl(*p, src->k.data())) return rc;
+  std::memcpy(p->timing, src->timing, sizeof p->timing);
+  *dst = p.release();
+  return NFFTM_OK;
+}
+
+void nfftm_plan_destroy(nfftm_plan p) {
+  if (!p) return;
+  (void)hipStreamSynchronize(p->stream);
+  delete p;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+"""ctypes binding of libnfftm.so (include/nfftm.h).
+
+The library is loaded from this package directory only; there is no CPU
+fallback, so importing the host API without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libnfftm.so")
+
+OK, ERR_ARG, ERR_DIM, ERR_UNSUPPORTED, ERR_HIP, ERR_FFT, ERR_OOM = range(7)
+F32, F64, MIXED = 0, 1, 2
+FULL, TENSOR, LINEAR, POLYNOMIAL = 1, 2, 3, 4
+KAISER_BESSEL = 0
+FLAG_BLOCKING, FLAG_TIMING = 1, 2
+
+# every symbol include/nfftm.h declares
+EXPORTS = (
+    "nfftm_plan_create", "nfftm_nodes", "nfftm_forward", "nfftm_adjoint",
+    "nfftm_deconvolve", "nfftm_deconvolve_transpose", "nfftm_convolve",
+    "nfftm_convolve_transpose", "nfftm_fft", "nfftm_sizes", "nfftm_grid",
+    "nfftm_timing", "nfftm_set_timing", "nfftm_set_stream", "nfftm_plan_copy",
+    "nfftm_plan_destroy", "nfftm_last_error", "nfftm_version",
+)
+
+_lib = None
+
+
+class LibraryMissing(ImportError):
+    pass
+
+
+def load():
+    """Load libnfftm.so and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise LibraryMissing(
+            f"{LIBPATH} is not built; run __graft_entry__.build() (make -C nfft.jl_amd/csrc)")
+    L = ctypes.CDLL(LIBPATH)
+    vp, i, i64, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    pi64 = ctypes.POINTER(ctypes.c_int64)
+    pd = ctypes.POINTER(ctypes.c_double)
+    L.nfftm_plan_create.argtypes = [ctypes.POINTER(vp), i, i, pi64, i64, pd, i, d, i, i, i, vp]
+    L.nfftm_nodes.argtypes = [vp, pd, i64]
+    for name in ("nfftm_forward", "nfftm_adjoint", "nfftm_deconvolve",
+                 "nfftm_deconvolve_transpose", "nfftm_convolve", "nfftm_convolve_transpose"):
+        getattr(L, name).argtypes = [vp, vp, vp]
+    L.nfftm_fft.argtypes = [vp, vp, i]
+    L.nfftm_sizes.argtypes = [vp, pi64, pi64, pd]
+    L.nfftm_grid.argtypes = [vp, ctypes.POINTER(vp)]
+    L.nfftm_timing.argtypes = [vp, pd]
+    L.nfftm_set_timing.argtypes = [vp, i]
+    L.nfftm_set_stream.argtypes = [vp, vp]
+    L.nfftm_plan_copy.argtypes = [vp, ctypes.POINTER(vp)]
+    L.nfftm_plan_destroy.argtypes = [vp]
+    L.nfftm_plan_destroy.restype = None
+    L.nfftm_last_error.restype = ctypes.c_char_p
+    L.nfftm_last_error.argtypes = []
+    L.nfftm_version.restype = ctypes.c_char_p
+    L.nfftm_version.argtypes = []
+    for name in EXPORTS:
+        if name not in ("nfftm_plan_destroy", "nfftm_last_error", "nfftm_version"):
+            getattr(L, name).restype = i
+    _lib = L
+    return L
