@@ -117,6 +117,16 @@ int nfftm_timing(nfftm_plan p, double t[7]);
  * per-call `timing=` keyword of mul! (src/implementation.jl:155-193). */
 int nfftm_set_timing(nfftm_plan p, int on);
 
+/* Kernel-level profiling without host synchronisation: while on, every
+ * stage launch of the transforms is bracketed by HIP events recorded on the
+ * plan's stream.  nfftm_profile_read synchronises, returns per stage the
+ * summed milliseconds and launch counts, and clears the record.  Stages:
+ * 0 deconvolve, 1 forward FFT, 2 interpolation, 3 grid clear, 4 spreading,
+ * 5 backward FFT, 6 deconvolve_transpose.  No counterpart in the reference
+ * (its TimingStats is host-timed); used by the benchmark. */
+int nfftm_profile(nfftm_plan p, int on);
+int nfftm_profile_read(nfftm_plan p, double ms[8], int64_t count[8]);
+
 /* Change the HIP stream the plan enqueues on. */
 int nfftm_set_stream(nfftm_plan p, void* hip_stream);
 

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_deconv(DArgs a, const typename cpx<R>::
     else if (x >= Nt0 - Na0) i0 = x - (Nt0 - Na0);
     if (inside && i0 >= 0) {
       const C v = f[fo + i0];
-      const double s = a.fac[0][i0] * facr;
+      const double s = a.fac[0][i0] * facr * a.oscale;
       out.x = (R)((double)v.x * s);
       out.y = (R)((double)v.y * s);
     }
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_deconv_t(DArgs a, const typename cpx<R>
     if (i0 >= N0) continue;
     const int gx = i0 < Na0 ? i0 - Na0 + a.Nt[0] : i0 - Na0;
     const C v = g[go + gx];
-    const double s = a.fac[0][i0] * facr;
+    const double s = a.fac[0][i0] * facr * a.oscale;
     C out;
     out.x = (R)((double)v.x * s);
     out.y = (R)((double)v.y * s);
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_spread_generic(KArgs a, const typename 
     const int cw = node_weights<R, W>(a, d, a.ks[(size_t)d * a.M + j], w[d]);
     base[d] = (long long)cw - a.m + 1;
   }
-  const C f = fhat[j];
+  const C f = C{fhat[j].x * (R)a.oscale, fhat[j].y * (R)a.oscale};
   R* g = (R*)grid;
   int l[MAXD];
   for (int d = 0; d < MAXD; ++d) l[d] = 0;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_interp_generic(KArgs a, const typename 
     }
     if (d == a.D) break;
   }
-  fhat[j] = C{sr, si};
+  fhat[j] = C{sr * (R)a.oscale, si * (R)a.oscale};
 }
 
 template <typename R>
@@ -225,8 +225,24 @@ hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s) {
 }
 
 size_t tile_lds_bytes(int D, bool dbl, int W, int tileElems) {
-  const int NB = D == 1 ? Batch<1>::NB : D == 2 ? Batch<2>::NB : Batch<3>::NB;
+  int NB;
+  if (dbl) NB = D == 1 ? Batch<1, double>::NB : D == 2 ? Batch<2, double>::NB : Batch<3, double>::NB;
+  else NB = D == 1 ? Batch<1, float>::NB : D == 2 ? Batch<2, float>::NB : Batch<3, float>::NB;
   return tile_layout_bytes(D, dbl ? sizeof(double) : sizeof(float), W, tileElems, NB);
+}
+
+hipError_t launch_fold(int D, bool dbl, const KArgs& a, hipStream_t s, void* grid) {
+  long long rows = 1;
+  for (int d = 1; d < D; ++d) rows *= a.Nt[d];
+  const dim3 g((a.Nt[0] + 255) / 256, (unsigned)std::min<long long>(rows, 65535)), blk(256);
+#define NFFTM_FOLD(DD)                                                                   \
+  if (D == DD) {                                                                         \
+    if (dbl) hipLaunchKernelGGL((k_fold<double, DD>), g, blk, 0, s, a, (double2*)grid);  \
+    else hipLaunchKernelGGL((k_fold<float, DD>), g, blk, 0, s, a, (float2*)grid);        \
+  }
+  NFFTM_FOLD(1) NFFTM_FOLD(2) NFFTM_FOLD(3)
+#undef NFFTM_FOLD
+  return hipGetLastError();
 }
 
 hipError_t launch_tile(int D, bool spread, bool dbl, int W, int NT, const KArgs& a, size_t lds, hipStream_t s,

@@ -7,8 +7,15 @@
 // its stencil halo ("padded tile", extent T + 2m - 1 per dimension) lives in
 // LDS:
 //   * spreading accumulates every node's (2m)^D footprint into the padded tile
-//     with LDS float atomics, then flushes it to the HBM grid with one
-//     coalesced pass of global atomics (halos overlap neighbouring tiles);
+//     with LDS atomics, then writes the padded tile with plain coalesced
+//     stores to a per-item scratch slot; a fold kernel then sums the (at most
+//     2^D, or 3^D for narrow tiles) padded tiles covering each grid point and
+//     writes every grid point once.  FP32 accumulates packed 32+32-bit
+//     fixed-point (re, im) pairs with one 64-bit integer LDS add per tap (on
+//     gfx950 the FP32 LDS float add costs ~40x an integer add); the scale is a
+//     power of two per work item from the strict bound sum_j |f_j| max(w)^D,
+//     so sums are exact integers, independent of order.  FP64 accumulates
+//     with the FP64 LDS add.
 //   * interpolation stages the padded tile from HBM once, then gathers every
 //     node's footprint from LDS.
 // Per node and dimension the FP64 index reduction of the reference
@@ -86,11 +93,19 @@ struct KArgs {
   const void* poly; // R[(2m+1) * 2m], column l holds coefficients of tap l
   double b;         // Kaiser-Bessel shape pi*(2 - 1/sigma)
   double wscale;    // power-of-two window normalisation
+  double oscale;    // output scale: 1 inside a transform; wscale^-D for the
+                    // standalone convolve!/convolve_transpose! entry points
   // nodes
   const double* ks;   // ks[d * M + j], shifted into [0, 1)
   const int* perm;    // sorted position -> caller's node index
   const int4* items;  // {tile, begin, end, 0}
   int nitems;
+  // spreading
+  double wmaxD;        // bound of any product of D window values
+  void* scratch;       // nitems padded tiles (dense, px fastest)
+  const int* tstart;   // items of tile t: [tstart[t], tstart[t+1])
+  int ntile_all;       // product of ntiles
+  int maxcand;         // covering (tile, offset) pairs per dim (fold)
 };
 
 // Window values of one node along dimension d.  Returns the wrapped cell
@@ -242,9 +257,9 @@ __host__ inline size_t tile_layout_bytes(int D, size_t sizeofR, int W, int tileE
 }
 
 // Nodes per phase-A batch.
-template <int D>
+template <int D, typename R>
 struct Batch {
-  static constexpr int NB = D == 3 ? 64 : D == 2 ? 128 : 256;
+  static constexpr int NB = D == 3 ? (sizeof(R) == 4 ? 256 : 128) : 256;
 };
 
 __device__ __forceinline__ void tile_origin(const KArgs& a, int D, int tile, int* t0) {
@@ -274,7 +289,10 @@ __device__ __forceinline__ void phase_a(const KArgs& a, const typename cpx<R>::T
     if (d == 0) {
       const int jj = a.perm[j];
       pj[n] = jj;
-      if (SPREAD) fv[n] = fhat[jj];
+      if (SPREAD) {
+        const typename cpx<R>::T v = fhat[jj];
+        fv[n] = typename cpx<R>::T{v.x * (R)a.oscale, v.y * (R)a.oscale};
+      }
     }
   }
 }
@@ -282,29 +300,84 @@ __device__ __forceinline__ void phase_a(const KArgs& a, const typename cpx<R>::T
 // ---------------------------------------------------------------------------
 // Spreading
 // ---------------------------------------------------------------------------
+// Packed fixed-point term: the 64-bit integer im * 2^32 + re (re, im signed
+// 32-bit).  Sums of such terms decode exactly while |sum re|, |sum im| < 2^31.
+__device__ __forceinline__ unsigned long long pack_term(float vr, float vi) {
+  const int re = __float2int_rn(vr), im = __float2int_rn(vi);
+  return ((unsigned long long)(unsigned)(im + (re >> 31)) << 32) | (unsigned)re;
+}
+
+template <typename R>
+struct SpreadCell;
+template <>
+struct SpreadCell<float> {  // one 64-bit integer per cell
+  static __device__ __forceinline__ void add(char* tile, int te, int idx, float vr, float vi) {
+    __hip_atomic_fetch_add(&((unsigned long long*)tile)[idx], pack_term(vr, vi), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  static __device__ __forceinline__ float2 get(const char* tile, int te, int idx, float inv) {
+    const long long s = (long long)((const unsigned long long*)tile)[idx];
+    const int re = (int)(unsigned)s;
+    const int im = (int)((s - (long long)re) >> 32);
+    return float2{(float)re * inv, (float)im * inv};
+  }
+};
+template <>
+struct SpreadCell<double> {  // two FP64 planes
+  static __device__ __forceinline__ void add(char* tile, int te, int idx, double vr, double vi) {
+    double* t = (double*)tile;
+    __hip_atomic_fetch_add(&t[idx], vr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&t[te + idx], vi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  static __device__ __forceinline__ double2 get(const char* tile, int te, int idx, double) {
+    const double* t = (const double*)tile;
+    return double2{t[idx], t[te + idx]};
+  }
+};
+
 template <typename R, int D, int W, int NT>
 __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<R>::T* __restrict__ fhat,
-                                                    typename cpx<R>::T* __restrict__ grid) {
+                                                    typename cpx<R>::T* __restrict__ unused) {
   using C = typename cpx<R>::T;
   using L = TileLayout<R, D, W>;
-  constexpr int NB = Batch<D>::NB;
+  constexpr int NB = Batch<D, R>::NB;
   constexpr int NW = NT / 64;
   extern __shared__ __align__(16) char smem[];
   const int item = xcd_item(a.nitems);
   if (item >= a.nitems) return;
   const int4 it = a.items[item];
-  R* tre = (R*)smem;
-  R* tim = tre + a.tileElems;
+  char* tile = smem;
   R* wts = (R*)(smem + L::wts_off(a.tileElems));
   C* fv = (C*)(smem + L::fv_off(a.tileElems, NB));
   int* loc = (int*)(smem + L::loc_off(a.tileElems, NB));
   int* pj = (int*)(smem + L::pj_off(a.tileElems, NB));
   int t0[3] = {0, 0, 0};
   tile_origin(a, D, it.x, t0);
-
-  for (int i = threadIdx.x; i < 2 * a.tileElems; i += NT) tre[i] = 0;
-
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+
+  // FP32: fixed-point scale s = 2^(30 - ceil(log2 B)), B = sum_j |f_j|_inf * wmaxD
+  R fscale = 1, inv = 1;
+  if (sizeof(R) == 4) {
+    R part = 0;
+    for (int j = it.y + threadIdx.x; j < it.z; j += NT) {
+      const C v = fhat[a.perm[j]];
+      part += fmax(fabs(v.x), fabs(v.y));
+    }
+    part = group_sum<64>(part);
+    R* red = wts;  // scratch before phase A
+    if (lane == 0) red[wave] = part;
+    __syncthreads();
+    R B = 0;
+    for (int w = 0; w < NW; ++w) B += red[w];
+    B = (R)((double)B * a.wmaxD * 1.0000001);
+    int e = 0;
+    if (B > 0) (void)frexp((double)B, &e);  // B < 2^e
+    fscale = (R)ldexp(1.0, 30 - e);
+    inv = (R)ldexp(1.0, e - 30);
+  }
+
+  for (int i = threadIdx.x; i < 2 * a.tileElems; i += NT) ((R*)tile)[i] = 0;
+
   for (int b0 = it.y; b0 < it.z; b0 += NB) {
     const int nb = min(NB, it.z - b0);
     __syncthreads();
@@ -320,14 +393,12 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
         const int bx = loc[n * 3], by = loc[n * 3 + 1], bz = loc[n * 3 + 2];
         for (int t = lane; t < W * W; t += 64) {
           const int ly = t / W, lx = t - ly * W;
-          const R wxy = w[lx] * w[W + ly];
+          const R wxy = w[lx] * w[W + ly] * fscale;
           const R vr = f.x * wxy, vi = f.y * wxy;
           const int base = (bx + lx) + a.Sx * (by + ly) + a.Sxy * bz;
 #pragma unroll
-          for (int lz = 0; lz < W; ++lz) {
-            lds_add(&tre[base + a.Sxy * lz], vr * w3[lz]);
-            lds_add(&tim[base + a.Sxy * lz], vi * w3[lz]);
-          }
+          for (int lz = 0; lz < W; ++lz)
+            SpreadCell<R>::add(tile, a.tileElems, base + a.Sxy * lz, vr * w3[lz], vi * w3[lz]);
         }
       }
     } else {
@@ -339,20 +410,17 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
         if (n < nb && lx < W) {
           const C f = fv[n];
           const R* w = wts + n * D * W;
-          const R w1 = w[lx];
+          const R w1 = w[lx] * fscale;
           const R vr = f.x * w1, vi = f.y * w1;
           if constexpr (D == 2) {
             const int base = (loc[n * 2] + lx) + a.Sx * loc[n * 2 + 1];
 #pragma unroll
             for (int ly = 0; ly < W; ++ly) {
               const R w2 = w[W + ly];
-              lds_add(&tre[base + a.Sx * ly], vr * w2);
-              lds_add(&tim[base + a.Sx * ly], vi * w2);
+              SpreadCell<R>::add(tile, a.tileElems, base + a.Sx * ly, vr * w2, vi * w2);
             }
           } else {
-            const int base = loc[n] + lx;
-            lds_add(&tre[base], vr);
-            lds_add(&tim[base], vi);
+            SpreadCell<R>::add(tile, a.tileElems, loc[n] + lx, vr, vi);
           }
         }
       }
@@ -360,8 +428,8 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
   }
   __syncthreads();
 
-  // flush the padded tile: global index (t0 - m + 1 + p) mod Ntilde per dim
-  R* g = (R*)grid;
+  // padded tile -> this item's scratch slot (dense, px fastest), plain stores
+  C* out = (C*)a.scratch + (size_t)item * a.padCells;
   for (int i = threadIdx.x; i < a.padCells; i += NT) {
     int px = i, py = 0, pz = 0;
     if (D >= 2) {
@@ -374,15 +442,76 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
         pz = q2;
       }
     }
-    const int li = px + a.Sx * py + a.Sxy * pz;
-    const R vr = tre[li], vi = tim[li];
-    if (vr != (R)0 || vi != (R)0) {
-      long long gi = wrapidx((long long)t0[0] - a.m + 1 + px, a.Nt[0]);
-      if (D >= 2) gi += wrapidx((long long)t0[1] - a.m + 1 + py, a.Nt[1]) * a.gstr[1];
-      if (D == 3) gi += wrapidx((long long)t0[2] - a.m + 1 + pz, a.Nt[2]) * a.gstr[2];
-      gbl_add(&g[2 * gi], vr);
-      gbl_add(&g[2 * gi + 1], vi);
+    out[i] = SpreadCell<R>::get(tile, a.tileElems, px + a.Sx * py + a.Sxy * pz, inv);
+  }
+}
+
+// Fold: grid point x sums, over the padded tiles covering it, the scratch
+// slots of every work item of those tiles.  Tile t's padded range is
+// [tT - m + 1, tT + T + m - 1] modulo Ntilde; with that range no longer than
+// Ntilde and 2T >= 2m - 1 (checked by the host) a point has at most 3
+// covering (tile, offset) pairs per dimension.
+__device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+__device__ __forceinline__ int cover_pairs(const KArgs& a, int d, int x, int* tl, int* pl) {
+  const int T = a.T[d], P = a.Pd[d], Nt = a.Nt[d], nt = a.ntiles[d], m1 = a.m - 1;
+  int n = 0;
+#pragma unroll
+  for (int k = -1; k <= 1; ++k) {
+    const int y = x + k * Nt;
+    int tlo = -floordiv(-(y - T - m1), T);  // ceil((y - T - m + 1) / T)
+    int thi = floordiv(y + m1, T);
+    tlo = max(tlo, 0);
+    thi = min(thi, nt - 1);
+    for (int t = tlo; t <= thi && n < 3; ++t) {
+      const int p = y - t * T + m1;
+      if (p >= 0 && p < P) {
+        tl[n] = t;
+        pl[n] = p;
+        ++n;
+      }
     }
+  }
+  return n;
+}
+
+template <typename R, int D>
+__global__ __launch_bounds__(256) void k_fold(KArgs a, typename cpx<R>::T* __restrict__ grid) {
+  using C = typename cpx<R>::T;
+  const C* scr = (const C*)a.scratch;
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int ny = D >= 2 ? a.Nt[1] : 1;
+  const int nrows = (D >= 2 ? a.Nt[1] : 1) * (D == 3 ? a.Nt[2] : 1);
+  int t1[3] = {0, 0, 0}, p1[3] = {0, 0, 0}, t2[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
+  int n1 = 1, n2 = 1;
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
+    const int y = row % ny, z = row / ny;
+    if (D >= 2) n1 = cover_pairs(a, 1, y, t1, p1);
+    if (D == 3) n2 = cover_pairs(a, 2, z, t2, p2);
+    if (x >= a.Nt[0]) continue;
+    int t0[3] = {0, 0, 0}, p0[3] = {0, 0, 0};
+    const int n0 = cover_pairs(a, 0, x, t0, p0);
+    R sr = 0, si = 0;
+#pragma unroll
+    for (int c2 = 0; c2 < 3; ++c2) {
+      if (c2 >= n2) break;
+#pragma unroll
+      for (int c1 = 0; c1 < 3; ++c1) {
+        if (c1 >= n1) break;
+#pragma unroll
+        for (int c0 = 0; c0 < 3; ++c0) {
+          if (c0 >= n0) break;
+          const int t = t0[c0] + a.ntiles[0] * (t1[c1] + a.ntiles[1] * t2[c2]);
+          const int off = p0[c0] + a.Pd[0] * (p1[c1] + a.Pd[1] * p2[c2]);
+          for (int itm = a.tstart[t]; itm < a.tstart[t + 1]; ++itm) {
+            const C v = scr[(size_t)itm * a.padCells + off];
+            sr += v.x;
+            si += v.y;
+          }
+        }
+      }
+    }
+    grid[(long long)row * a.Nt[0] + x] = C{sr, si};
   }
 }
 
@@ -394,7 +523,7 @@ __global__ __launch_bounds__(NT) void k_interp_tile(KArgs a, const typename cpx<
                                                     typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
   using L = TileLayout<R, D, W>;
-  constexpr int NB = Batch<D>::NB;
+  constexpr int NB = Batch<D, R>::NB;
   constexpr int NW = NT / 64;
   extern __shared__ __align__(16) char smem[];
   const int item = xcd_item(a.nitems);
@@ -456,7 +585,7 @@ __global__ __launch_bounds__(NT) void k_interp_tile(KArgs a, const typename cpx<
         }
         ar = group_sum<64>(ar);
         ai = group_sum<64>(ai);
-        if (lane == 0) fhat[pj[n]] = C{ar, ai};
+        if (lane == 0) fhat[pj[n]] = C{ar * (R)a.oscale, ai * (R)a.oscale};
       }
     } else {
       constexpr int G = LanesPerNode<W>::value;
@@ -485,7 +614,7 @@ __global__ __launch_bounds__(NT) void k_interp_tile(KArgs a, const typename cpx<
         }
         ar = group_sum<G>(ar);
         ai = group_sum<G>(ai);
-        if (lx == 0 && n < nb) fhat[pj[n]] = C{ar, ai};
+        if (lx == 0 && n < nb) fhat[pj[n]] = C{ar * (R)a.oscale, ai * (R)a.oscale};
       }
     }
   }

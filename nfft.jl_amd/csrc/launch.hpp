@@ -13,6 +13,7 @@ struct DArgs {
   long long nstr[MAXD], gstr[MAXD];
   const double* fac[MAXD];  // 1/psi_hat per dim, FP64, N[d] entries
   long long P, Q;
+  double oscale;  // 1 inside a transform; wscale^D for standalone deconvolve!
 };
 
 // LDS bytes of a tile kernel.
@@ -35,5 +36,8 @@ hipError_t launch_generic(bool spread, bool dbl, int W, const KArgs& a, hipStrea
 hipError_t launch_deconv(bool dbl, const DArgs& a, hipStream_t s, const void* f, void* g);
 hipError_t launch_deconv_t(bool dbl, const DArgs& a, hipStream_t s, const void* g, void* f);
 hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s);
+
+// Sum the per-item padded tiles in a.scratch into the grid (D <= 3).
+hipError_t launch_fold(int D, bool dbl, const KArgs& a, hipStream_t s, void* grid);
 
 }  // namespace nfftm

@@ -284,7 +284,10 @@ struct nfftm_plan_s {
   DevBuf<double> d_deconv, d_ks;
   DevBuf<int> d_perm;
   DevBuf<int4> d_items;
+  DevBuf<int> d_tstart;
+  DevBuf<char> d_scratch;
   int nitems = 0;
+  double wmax = 1.0;  // bound of one (normalised) window value
 
   // tiling
   int T[3] = {1, 1, 1}, ntiles[3] = {1, 1, 1}, Pd[3] = {1, 1, 1}, Sx = 1, Sxy = 1, tileElems = 1, NT = 256;
@@ -292,12 +295,55 @@ struct nfftm_plan_s {
 
   std::unique_ptr<FFTStage> fft;
   double timing[7] = {0};
+  // kernel-level profiling (event pairs recorded without host sync)
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
+  std::vector<int> prof_stage;
+  size_t prof_used = 0;
+  double prof_ms[8] = {0};
+  int64_t prof_cnt[8] = {0};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<double> k;  // caller's nodes (D x M), kept for copy()
 
   ~nfftm_plan_s() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& pr : prof_ev) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  }
+
+  // fold recorded event pairs into the per-stage sums (synchronises)
+  int prof_drain() {
+    if (prof_used == 0) return NFFTM_OK;
+    hipError_t e = hipEventSynchronize(prof_ev[prof_used - 1].second);
+    if (e != hipSuccess) return NFFTM_ERR_HIP;
+    for (size_t i = 0; i < prof_used; ++i) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, prof_ev[i].first, prof_ev[i].second) != hipSuccess) return NFFTM_ERR_HIP;
+      prof_ms[prof_stage[i]] += ms;
+      prof_cnt[prof_stage[i]] += 1;
+    }
+    prof_used = 0;
+    return NFFTM_OK;
+  }
+
+  // returns the event pair to record around a stage launch, or null
+  std::pair<hipEvent_t, hipEvent_t>* prof_slot(int stage) {
+    if (!profiling) return nullptr;
+    if (prof_used == prof_ev.size()) {
+      if (prof_ev.size() >= 4096) {
+        if (prof_drain() != NFFTM_OK) return nullptr;
+      } else {
+        std::pair<hipEvent_t, hipEvent_t> pr;
+        if (hipEventCreate(&pr.first) != hipSuccess || hipEventCreate(&pr.second) != hipSuccess) return nullptr;
+        prof_ev.push_back(pr);
+        prof_stage.push_back(0);
+      }
+    }
+    prof_stage[prof_used] = stage;
+    return &prof_ev[prof_used++];
   }
 
   KArgs kargs() const {
@@ -332,10 +378,15 @@ struct nfftm_plan_s {
     a.poly = d_poly.p;
     a.b = b;
     a.wscale = wscale;
+    a.oscale = 1.0;
     a.ks = d_ks.p;
     a.perm = d_perm.p;
     a.items = d_items.p;
     a.nitems = nitems;
+    a.wmaxD = std::pow(wmax, (double)D);
+    a.scratch = d_scratch.p;
+    a.tstart = d_tstart.p;
+    a.ntile_all = ntiles[0] * ntiles[1] * ntiles[2];
     return a;
   }
 
@@ -357,11 +408,36 @@ struct nfftm_plan_s {
     }
     a.P = P;
     a.Q = Q;
+    a.oscale = 1.0;
     return a;
   }
 };
 
 namespace {
+
+// Largest number of padded tiles covering one grid point along one dimension
+// (tile t covers [tT - m + 1, tT + T + m - 1] mod Ntilde): exact near the
+// periodic wrap (where a partial last tile brings origins closer than T),
+// analytic in the interior.
+int max_cover(i64 T, i64 P, i64 Nt, int m) {
+  const i64 nt = (Nt + T - 1) / T;
+  int best = (int)std::min<i64>((T + 2 * m - 2) / T + 1, nt);
+  auto count = [&](i64 x) {
+    int c = 0;
+    for (i64 t = 0; t < nt; ++t) {
+      i64 p = ((x - t * T + m - 1) % Nt + Nt) % Nt;
+      for (; p < P; p += Nt) ++c;
+    }
+    return c;
+  };
+  if (nt > 4096) {  // only the wrap neighbourhood can exceed the interior bound
+    for (i64 x = 0; x < std::min(Nt, 2 * P); ++x) best = std::max(best, count(x));
+    for (i64 x = std::max<i64>(0, Nt - 2 * P); x < Nt; ++x) best = std::max(best, count(x));
+  } else {
+    for (i64 x = 0; x < Nt; ++x) best = std::max(best, count(x));
+  }
+  return best;
+}
 
 // Tile geometry of the gridding kernels (D <= 3).  Sx is the padded row
 // extent rounded to 8 (mod 16) when that costs at most one element, which
@@ -412,6 +488,18 @@ void choose_tiles(nfftm_plan_s& p) {
   }
 }
 
+// The tiled path needs the padded range within one period and at most 3
+// covering tiles per dimension (the fold kernel's bound); tiny grids use the
+// node-parallel kernels.
+bool tiles_ok(const nfftm_plan_s& p) {
+  if (p.lds > 160 * 1024) return false;
+  for (int d = 0; d < p.D; ++d) {
+    if (p.Pd[d] > p.Nt[d]) return false;
+    if (max_cover(p.T[d], p.Pd[d], p.Nt[d], p.m) > 3) return false;
+  }
+  return true;
+}
+
 // Window normalisation by an exact power of two so the FP32 weights stay near
 // 1 (the unnormalised psi(0) reaches 1e15 at m = 8); the deconvolution factors
 // carry the inverse power, so the transform is unchanged.
@@ -449,6 +537,7 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
   std::vector<double> ks((size_t)D * M);
   std::vector<int> perm;
   std::vector<int4> items;
+  std::vector<int> tstart;
   auto shifted = [&](i64 j, int d) {
     double x = k[d + (size_t)D * j];
     if (x < 0.0) x += 1.0;
@@ -486,12 +575,15 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
       for (int d = 0; d < D; ++d) ks[(size_t)d * M + s] = shifted(perm[s], d);
     // work items: chunks of at most CH nodes of one tile
     const i64 CH = D == 3 ? 1024 : D == 2 ? 2048 : 4096;
+    tstart.assign(ntile + 1, 0);
     for (i64 t = 0; t < ntile; ++t) {
+      tstart[t] = (int)items.size();
       for (i64 b0 = cnt[t]; b0 < cnt[t + 1]; b0 += CH) {
         const i64 e0 = std::min(cnt[t + 1], b0 + CH);
         items.push_back(int4{(int)t, (int)b0, (int)e0, 0});
       }
     }
+    tstart[ntile] = (int)items.size();
   }
   p.nitems = (int)items.size();
   HIPCHK(p.d_ks.alloc(ks.size()));
@@ -502,6 +594,12 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
     HIPCHK(hipMemcpy(p.d_perm.p, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice));
   if (!items.empty())
     HIPCHK(hipMemcpy(p.d_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+  if (p.tiled) {
+    HIPCHK(p.d_tstart.alloc(tstart.size()));
+    HIPCHK(hipMemcpy(p.d_tstart.p, tstart.data(), tstart.size() * sizeof(int), hipMemcpyHostToDevice));
+    const size_t pad = (size_t)p.Pd[0] * p.Pd[1] * p.Pd[2];
+    HIPCHK(p.d_scratch.alloc(std::max<size_t>(1, items.size()) * pad * p.celem()));
+  }
   return NFFTM_OK;
 }
 
@@ -509,6 +607,9 @@ int build_tables(nfftm_plan_s& p) {
   const int m = p.m;
   p.b = M_PI * (2.0 - 1.0 / p.sigma);
   p.wscale = window_scale(m, p.b);
+  // bound of one window value: psi(0) for the table and the exact window; the
+  // fitted polynomials stay within a hair of psi (5 % slack)
+  p.wmax = kb_psi(0.0, m, p.b) * p.wscale * (p.precompute == NFFTM_LINEAR || p.precompute == NFFTM_FULL ? 1.0 : 1.05);
   hipError_t e = hipSuccess;
   if (p.precompute == NFFTM_LINEAR) {
     // precomputeLinInterp (src/precomputation.jl:291-300): K+2 samples, step m/K
@@ -540,20 +641,46 @@ int build_tables(nfftm_plan_s& p) {
   return NFFTM_OK;
 }
 
-int run_conv(nfftm_plan_s& p, bool spread, const void* in, void* out) {
-  const KArgs a = p.kargs();
+// standalone: called as convolve!/convolve_transpose! on a caller grid, so the
+// window normalisation is undone to give the reference's unnormalised values.
+struct ProfScope {
+  std::pair<hipEvent_t, hipEvent_t>* s;
+  hipStream_t st;
+  ProfScope(nfftm_plan_s& p, int stage) : s(p.prof_slot(stage)), st(p.stream) {
+    if (s) (void)hipEventRecord(s->first, st);
+  }
+  ~ProfScope() {
+    if (s) (void)hipEventRecord(s->second, st);
+  }
+};
+
+int run_conv(nfftm_plan_s& p, bool spread, const void* in, void* out, bool standalone = false) {
+  KArgs a = p.kargs();
+  if (standalone) a.oscale = std::pow(p.wscale, -(double)p.D);
   hipError_t e;
-  if (spread) HIPCHK(launch_fill0(out, (size_t)p.Q * p.celem(), p.stream));
-  if (p.tiled)
-    e = launch_tile(p.D, spread, p.dbl(), p.W, p.NT, a, p.lds, p.stream, in, out);
-  else
-    e = launch_generic(spread, p.dbl(), p.W, a, p.stream, in, out);
+  if (spread && !p.tiled) {
+    ProfScope ps(p, 3);
+    HIPCHK(launch_fill0(out, (size_t)p.Q * p.celem(), p.stream));
+  }
+  {
+    ProfScope ps(p, spread ? 4 : 2);
+    if (p.tiled)
+      e = launch_tile(p.D, spread, p.dbl(), p.W, p.NT, a, p.lds, p.stream, in, spread ? nullptr : out);
+    else
+      e = launch_generic(spread, p.dbl(), p.W, a, p.stream, in, out);
+  }
   HIPCHK(e);
+  if (spread && p.tiled) {
+    ProfScope ps(p, 3);
+    HIPCHK(launch_fold(p.D, p.dbl(), a, p.stream, out));
+  }
   return NFFTM_OK;
 }
 
-int run_deconv(nfftm_plan_s& p, bool fwd, const void* in, void* out) {
-  const DArgs a = p.dargs();
+int run_deconv(nfftm_plan_s& p, bool fwd, const void* in, void* out, bool standalone = false) {
+  DArgs a = p.dargs();
+  if (standalone) a.oscale = std::pow(p.wscale, (double)p.D);
+  ProfScope ps(p, fwd ? 0 : 6);
   if (fwd) HIPCHK(launch_deconv(p.dbl(), a, p.stream, in, out));
   else HIPCHK(launch_deconv_t(p.dbl(), a, p.stream, in, out));
   return NFFTM_OK;
@@ -596,7 +723,10 @@ int create_impl(nfftm_plan_s& p, const double* k) {
     if (p.Nt[d] > 0x7fffffff)
       return fail(NFFTM_ERR_UNSUPPORTED, "oversampled size %lld exceeds 2^31-1", (long long)p.Nt[d]);
   p.tiled = (p.flags & NFFTM_FLAG_BLOCKING) && p.D <= 3;
-  if (p.tiled) choose_tiles(p);
+  if (p.tiled) {
+    choose_tiles(p);
+    p.tiled = tiles_ok(p);
+  }
   if (int rc = build_tables(p)) return rc;
   if (int rc = set_nodes(p, k, p.M)) return rc;
   HIPCHK(p.d_grid.alloc((size_t)p.Q * p.celem()));
@@ -671,7 +801,10 @@ int nfftm_forward(nfftm_plan p, const void* f, void* fhat) {
   if (int rc = t.mark(0)) return rc;
   if (int rc = run_deconv(*p, true, f, p->d_grid.p)) return rc;
   if (int rc = t.mark(1)) return rc;
-  if (int rc = p->fft->exec(true, p->d_grid.p, p->celem(), p->stream)) return rc;
+  {
+    ProfScope ps(*p, 1);
+    if (int rc = p->fft->exec(true, p->d_grid.p, p->celem(), p->stream)) return rc;
+  }
   if (int rc = t.mark(2)) return rc;
   if (p->M) {
     if (int rc = run_conv(*p, false, p->d_grid.p, fhat)) return rc;
@@ -688,7 +821,10 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
   if (int rc = t.mark(0)) return rc;
   if (int rc = run_conv(*p, true, fhat, p->d_grid.p)) return rc;
   if (int rc = t.mark(1)) return rc;
-  if (int rc = p->fft->exec(false, p->d_grid.p, p->celem(), p->stream)) return rc;
+  {
+    ProfScope ps(*p, 5);
+    if (int rc = p->fft->exec(false, p->d_grid.p, p->celem(), p->stream)) return rc;
+  }
   if (int rc = t.mark(2)) return rc;
   if (int rc = run_deconv(*p, false, p->d_grid.p, f)) return rc;
   if (int rc = t.mark(3)) return rc;
@@ -698,26 +834,26 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
 int nfftm_deconvolve(nfftm_plan p, const void* f, void* g) {
   g_err.clear();
   if (!p || !f || !g) return fail(NFFTM_ERR_ARG, "NULL argument");
-  return run_deconv(*p, true, f, g);
+  return run_deconv(*p, true, f, g, true);
 }
 
 int nfftm_deconvolve_transpose(nfftm_plan p, const void* g, void* f) {
   g_err.clear();
   if (!p || !f || !g) return fail(NFFTM_ERR_ARG, "NULL argument");
-  return run_deconv(*p, false, g, f);
+  return run_deconv(*p, false, g, f, true);
 }
 
 int nfftm_convolve(nfftm_plan p, const void* g, void* fhat) {
   g_err.clear();
   if (!p || !g || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL argument");
   if (!p->M) return NFFTM_OK;
-  return run_conv(*p, false, g, fhat);
+  return run_conv(*p, false, g, fhat, true);
 }
 
 int nfftm_convolve_transpose(nfftm_plan p, const void* fhat, void* g) {
   g_err.clear();
   if (!p || !g || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL argument");
-  return run_conv(*p, true, fhat, g);
+  return run_conv(*p, true, fhat, g, true);
 }
 
 int nfftm_fft(nfftm_plan p, void* g, int direction) {
@@ -756,6 +892,26 @@ int nfftm_set_timing(nfftm_plan p, int on) {
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
   if (on) p->flags |= NFFTM_FLAG_TIMING;
   else p->flags &= ~NFFTM_FLAG_TIMING;
+  return NFFTM_OK;
+}
+
+int nfftm_profile(nfftm_plan p, int on) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  p->profiling = on != 0;
+  return NFFTM_OK;
+}
+
+int nfftm_profile_read(nfftm_plan p, double ms[8], int64_t count[8]) {
+  g_err.clear();
+  if (!p || !ms || !count) return fail(NFFTM_ERR_ARG, "NULL argument");
+  if (p->prof_drain() != NFFTM_OK) return fail(NFFTM_ERR_HIP, "profile event read failed");
+  for (int i = 0; i < 8; ++i) {
+    ms[i] = p->prof_ms[i];
+    count[i] = p->prof_cnt[i];
+    p->prof_ms[i] = 0;
+    p->prof_cnt[i] = 0;
+  }
   return NFFTM_OK;
 }
 

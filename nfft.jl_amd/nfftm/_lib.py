@@ -22,7 +22,8 @@ EXPORTS = (
     "nfftm_plan_create", "nfftm_nodes", "nfftm_forward", "nfftm_adjoint",
     "nfftm_deconvolve", "nfftm_deconvolve_transpose", "nfftm_convolve",
     "nfftm_convolve_transpose", "nfftm_fft", "nfftm_sizes", "nfftm_grid",
-    "nfftm_timing", "nfftm_set_timing", "nfftm_set_stream", "nfftm_plan_copy",
+    "nfftm_timing", "nfftm_set_timing", "nfftm_profile", "nfftm_profile_read",
+    "nfftm_set_stream", "nfftm_plan_copy",
     "nfftm_plan_destroy", "nfftm_last_error", "nfftm_version",
 )
 
@@ -55,6 +56,8 @@ def load():
     L.nfftm_grid.argtypes = [vp, ctypes.POINTER(vp)]
     L.nfftm_timing.argtypes = [vp, pd]
     L.nfftm_set_timing.argtypes = [vp, i]
+    L.nfftm_profile.argtypes = [vp, i]
+    L.nfftm_profile_read.argtypes = [vp, pd, pi64]
     L.nfftm_set_stream.argtypes = [vp, vp]
     L.nfftm_plan_copy.argtypes = [vp, ctypes.POINTER(vp)]
     L.nfftm_plan_destroy.argtypes = [vp]
