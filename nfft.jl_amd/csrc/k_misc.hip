@@ -231,7 +231,8 @@ size_t tile_lds_bytes(int D, bool dbl, int W, int tileElems) {
   return tile_layout_bytes(D, dbl ? sizeof(double) : sizeof(float), W, tileElems, NB);
 }
 
-hipError_t launch_fold(int D, bool dbl, const KArgs& a, hipStream_t s, void* grid) {
+hipError_t launch_fold(int D, bool dbl, int W, const KArgs& a, hipStream_t s, void* grid) {
+  if (D == 3) return launch_fold_d3(dbl, W, a, s, grid);
   long long rows = 1;
   for (int d = 1; d < D; ++d) rows *= a.Nt[d];
   const dim3 g((a.Nt[0] + 255) / 256, (unsigned)std::min<long long>(rows, 65535)), blk(256);
@@ -240,7 +241,7 @@ hipError_t launch_fold(int D, bool dbl, const KArgs& a, hipStream_t s, void* gri
     if (dbl) hipLaunchKernelGGL((k_fold<double, DD>), g, blk, 0, s, a, (double2*)grid);  \
     else hipLaunchKernelGGL((k_fold<float, DD>), g, blk, 0, s, a, (float2*)grid);        \
   }
-  NFFTM_FOLD(1) NFFTM_FOLD(2) NFFTM_FOLD(3)
+  NFFTM_FOLD(1) NFFTM_FOLD(2)
 #undef NFFTM_FOLD
   return hipGetLastError();
 }

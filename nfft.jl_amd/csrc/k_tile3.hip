@@ -1,12 +1,425 @@
-// Tile kernels for D = 3 (see kernels.hpp).
+// k_tile3.hip -- 3-D gridding kernels with compile-time tile geometry.
+//
+// Same algorithm as the generic tile kernels of kernels.hpp (padded tile in
+// LDS, nodes bin-sorted into tiles, one workgroup per work item), specialised
+// for D = 3 so that the LDS strides are constants (immediate ds offsets, no
+// integer divisions), with 1024-thread workgroups for latency hiding and two
+// nodes in flight per wavefront.
 #include "launch.hpp"
-#define TILE_D 3
-#include "tile_launch.inc"
 
 namespace nfftm {
-hipError_t launch_tile_d3(bool spread, bool dbl, int W, int NT, const KArgs& a, size_t lds,
-                          hipStream_t s, const void* in, void* out) {
-  return dbl ? tile_w<double>(spread, W, NT, a, lds, s, in, out)
-             : tile_w<float>(spread, W, NT, a, lds, s, in, out);
+
+// Tile edge per precision and stencil width W = 2m, the largest that keeps
+// the padded tile (T + W - 1)^3 plus the per-batch node records within
+// 160 KiB of LDS.
+template <typename R, int W>
+struct Geo3 {
+  static constexpr int T = sizeof(R) == 4 ? (W <= 10 ? 16 : W <= 12 ? 12 : W <= 14 ? 10 : 8)
+                                          : (W <= 8 ? 12 : W <= 12 ? 8 : W <= 14 ? 6 : 4);
+  static constexpr int P = T + W - 1;
+  static constexpr int SX = (P % 16 == 7) ? P + 1 : P;  // W = 8: 8x8 lanes on distinct banks
+  static constexpr int SXY = SX * P;
+  static constexpr int CELLS = SXY * P;                 // elements of one LDS plane
+  static constexpr int PAD = P * P * P;                 // dense padded cells
+  static constexpr size_t TILE_BYTES = 2 * (size_t)CELLS * sizeof(R);
+  static constexpr size_t rec_bytes(int nb) {
+    return (size_t)nb * 3 * W * sizeof(R) + (size_t)nb * 2 * sizeof(R) + (size_t)nb * 8;
+  }
+  static constexpr int NB = TILE_BYTES + rec_bytes(256) + 64 <= 163840   ? 256
+                            : TILE_BYTES + rec_bytes(128) + 64 <= 163840 ? 128
+                                                                         : 64;
+  static constexpr size_t WTS_OFF = TILE_BYTES;
+  static constexpr size_t FV_OFF = (WTS_OFF + (size_t)NB * 3 * W * sizeof(R) + 15) & ~(size_t)15;
+  static constexpr size_t LOC_OFF = FV_OFF + (size_t)NB * 2 * sizeof(R);
+  static constexpr size_t PJ_OFF = LOC_OFF + (size_t)NB * 4;
+  static constexpr size_t LDS = PJ_OFF + (size_t)NB * 4;
+  static_assert(LDS <= 163840, "3-D tile does not fit LDS");
+};
+
+__device__ __forceinline__ int wrap1(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
+
+// Sum over the 64 lanes; the result is valid in lane 63.
+template <typename R>
+__device__ __forceinline__ R wave_total(R v) {
+  v += dpp<0xB1>(v);   // quad_perm(1,0,3,2)
+  v += dpp<0x4E>(v);   // quad_perm(2,3,0,1)
+  v += dpp<0x141>(v);  // row_half_mirror
+  v += dpp<0x140>(v);  // row_mirror: every lane holds its row's sum
+  v += dpp_rows<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_rows<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
+  return v;
 }
+
+// Phase A for D = 3: node records of nodes [b0, b0 + nb) of the item.
+template <typename R, int W, int NT, bool SPREAD>
+__device__ __forceinline__ void phase_a3(const KArgs& a, const typename cpx<R>::T* __restrict__ fhat, int b0,
+                                         int nb, const int* t0, R* wts, typename cpx<R>::T* fv, int* loc,
+                                         int* pj, R fscale) {
+  for (int q = threadIdx.x; q < nb * 3; q += NT) {
+    const int n = q / 3, d = q - n * 3;
+    const int j = b0 + n;
+    R w[W];
+    const int cw = node_weights<R, W>(a, d, a.ks[(size_t)d * a.M + j], w);
+#pragma unroll
+    for (int l = 0; l < W; ++l) wts[(n * 3 + d) * W + l] = w[l];
+    // packed local offsets: 8 bits per dimension (T + W <= 32)
+    atomicOr(&loc[n], (cw - t0[d]) << (8 * d));
+    if (d == 0) {
+      const int jj = a.perm[j];
+      pj[n] = jj;
+      if (SPREAD) {
+        const typename cpx<R>::T v = fhat[jj];
+        fv[n] = typename cpx<R>::T{v.x * fscale * (R)a.oscale, v.y * fscale * (R)a.oscale};
+      }
+    }
+  }
+}
+
+template <typename R, int W, int NT>
+__global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
+                                                typename cpx<R>::T* __restrict__ fhat) {
+  using C = typename cpx<R>::T;
+  using G = Geo3<R, W>;
+  constexpr int NW = NT / 64;
+  extern __shared__ __align__(16) char smem[];
+  const int item = xcd_item(a.nitems);
+  if (item >= a.nitems) return;
+  const int4 it = a.items[item];
+  C* tile = (C*)smem;
+  R* wts = (R*)(smem + G::WTS_OFF);
+  int* loc = (int*)(smem + G::LOC_OFF);
+  int* pj = (int*)(smem + G::PJ_OFF);
+  int t0[3];
+  {
+    const int tx = it.x % a.ntiles[0], q = it.x / a.ntiles[0];
+    t0[0] = tx * G::T;
+    t0[1] = (q % a.ntiles[1]) * G::T;
+    t0[2] = (q / a.ntiles[1]) * G::T;
+  }
+  const int Nx = a.Nt[0], Ny = a.Nt[1], Nz = a.Nt[2];
+  const long long sy = a.gstr[1], sz = a.gstr[2];
+
+  // stage the padded tile, one padded row per 32 lanes
+  {
+    const int px = threadIdx.x & 31;
+    const int gx = wrap1(t0[0] - a.m + 1 + px, Nx);
+    for (int r = threadIdx.x >> 5; r < G::P * G::P; r += NT / 32) {
+      const int py = r % G::P, pz = r / G::P;
+      if (px < G::P) {
+        const long long g = (long long)wrap1(t0[2] - a.m + 1 + pz, Nz) * sz +
+                            (long long)wrap1(t0[1] - a.m + 1 + py, Ny) * sy + gx;
+        tile[px + G::SX * py + G::SXY * pz] = grid[g];
+      }
+    }
+  }
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const R osc = (R)a.oscale;
+  for (int b0 = it.y; b0 < it.z; b0 += G::NB) {
+    const int nb = min(G::NB, it.z - b0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nb; q += NT) loc[q] = 0;
+    __syncthreads();
+    phase_a3<R, W, NT, false>(a, nullptr, b0, nb, t0, wts, nullptr, loc, pj, (R)1);
+    __syncthreads();
+    for (int n = 2 * wave; n < nb; n += 2 * NW) {
+      const bool two = n + 1 < nb;
+      R acc[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int nn = two ? n + u : n;
+        const R* w = wts + nn * 3 * W;
+        R w3[W];
+#pragma unroll
+        for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
+        const int pk = loc[nn];
+        const int bx = pk & 255, by = (pk >> 8) & 255, bz = (pk >> 16) & 255;
+#pragma unroll
+        for (int t0l = 0; t0l < W * W; t0l += 64) {
+          const int t = t0l + lane;
+          if (t < W * W) {
+            const int ly = t / W, lx = t - ly * W;
+            const C* base = tile + (bx + lx) + G::SX * (by + ly) + G::SXY * bz;
+            R sr = 0, si = 0;
+#pragma unroll
+            for (int lz = 0; lz < W; ++lz) {
+              const C v = base[G::SXY * lz];
+              sr = fma(w3[lz], v.x, sr);
+              si = fma(w3[lz], v.y, si);
+            }
+            const R wxy = w[lx] * w[W + ly];
+            acc[u][0] = fma(wxy, sr, acc[u][0]);
+            acc[u][1] = fma(wxy, si, acc[u][1]);
+          }
+        }
+      }
+      const R s0 = wave_total(acc[0][0]), s1 = wave_total(acc[0][1]);
+      const R s2 = wave_total(acc[1][0]), s3 = wave_total(acc[1][1]);
+      if (lane == 63) {
+        fhat[pj[n]] = C{s0 * osc, s1 * osc};
+        if (two) fhat[pj[n + 1]] = C{s2 * osc, s3 * osc};
+      }
+    }
+  }
+}
+
+template <typename R, int W, int NT>
+__global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
+  using C = typename cpx<R>::T;
+  using G = Geo3<R, W>;
+  constexpr int NW = NT / 64;
+  extern __shared__ __align__(16) char smem[];
+  const int item = xcd_item(a.nitems);
+  if (item >= a.nitems) return;
+  const int4 it = a.items[item];
+  char* tile = smem;
+  R* wts = (R*)(smem + G::WTS_OFF);
+  C* fv = (C*)(smem + G::FV_OFF);
+  int* loc = (int*)(smem + G::LOC_OFF);
+  int* pj = (int*)(smem + G::PJ_OFF);
+  int t0[3];
+  {
+    const int tx = it.x % a.ntiles[0], q = it.x / a.ntiles[0];
+    t0[0] = tx * G::T;
+    t0[1] = (q % a.ntiles[1]) * G::T;
+    t0[2] = (q / a.ntiles[1]) * G::T;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+
+  // FP32: fixed-point scale s = 2^(30 - e), B = sum_j |f_j|_inf * wmaxD < 2^e
+  R fscale = 1, inv = 1;
+  if (sizeof(R) == 4) {
+    R part = 0;
+    for (int j = it.y + threadIdx.x; j < it.z; j += NT) {
+      const C v = fhat[a.perm[j]];
+      part += fmax(fabs(v.x), fabs(v.y));
+    }
+    part = wave_total(part);
+    R* red = wts;
+    if (lane == 63) red[wave] = part;
+    __syncthreads();
+    R B = 0;
+    for (int w = 0; w < NW; ++w) B += red[w];
+    B = (R)((double)B * a.oscale * a.wmaxD * 1.0000001);
+    int e = 0;
+    if (B > 0) (void)frexp((double)B, &e);
+    fscale = (R)ldexp(1.0, 30 - e);
+    inv = (R)ldexp(1.0, e - 30);
+  }
+  for (int i = threadIdx.x; i < 2 * G::CELLS; i += NT) ((R*)tile)[i] = 0;
+
+  for (int b0 = it.y; b0 < it.z; b0 += G::NB) {
+    const int nb = min(G::NB, it.z - b0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nb; q += NT) loc[q] = 0;
+    __syncthreads();
+    phase_a3<R, W, NT, true>(a, fhat, b0, nb, t0, wts, fv, loc, pj, fscale);
+    __syncthreads();
+    for (int n = wave; n < nb; n += NW) {
+      const C f = fv[n];
+      const R* w = wts + n * 3 * W;
+      R w3[W];
+#pragma unroll
+      for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
+      const int pk = loc[n];
+      const int bx = pk & 255, by = (pk >> 8) & 255, bz = (pk >> 16) & 255;
+#pragma unroll
+      for (int t0l = 0; t0l < W * W; t0l += 64) {
+        const int t = t0l + lane;
+        if (t < W * W) {
+          const int ly = t / W, lx = t - ly * W;
+          const R wxy = w[lx] * w[W + ly];
+          const R vr = f.x * wxy, vi = f.y * wxy;
+          const int base = (bx + lx) + G::SX * (by + ly) + G::SXY * bz;
+          if constexpr (sizeof(R) == 4) {
+            unsigned long long* tl = (unsigned long long*)tile + base;
+#pragma unroll
+            for (int lz = 0; lz < W; ++lz) {
+              int re, im;
+              asm volatile("v_cvt_rpi_i32_f32 %0, %1" : "=v"(re) : "v"((float)(vr * w3[lz])));
+              asm volatile("v_cvt_rpi_i32_f32 %0, %1" : "=v"(im) : "v"((float)(vi * w3[lz])));
+              const unsigned long long term =
+                  ((unsigned long long)(unsigned)(im + (re >> 31)) << 32) | (unsigned)re;
+              __hip_atomic_fetch_add(tl + G::SXY * lz, term, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+          } else {
+            double* tr = (double*)tile + base;
+#pragma unroll
+            for (int lz = 0; lz < W; ++lz) {
+              __hip_atomic_fetch_add(tr + G::SXY * lz, (double)(vr * w3[lz]), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+              __hip_atomic_fetch_add(tr + G::CELLS + G::SXY * lz, (double)(vi * w3[lz]), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // padded tile -> this item's scratch slot, one padded row per 32 lanes
+  C* out = (C*)a.scratch + (size_t)item * G::PAD;
+  const int px = threadIdx.x & 31;
+  for (int r = threadIdx.x >> 5; r < G::P * G::P; r += NT / 32) {
+    if (px < G::P) {
+      const int li = px + G::SX * (r % G::P) + G::SXY * (r / G::P);
+      out[px + G::P * r] = SpreadCell<R>::get(tile, G::CELLS, li, inv);
+    }
+  }
+}
+
+// Fold for D = 3 (see k_fold in kernels.hpp); T, P compile-time.
+template <int T, int P>
+__device__ __forceinline__ int cover3(int x, int Nt, int nt, int m1, int* tl, int* pl) {
+  int n = 0;
+#pragma unroll
+  for (int k = -1; k <= 1; ++k) {
+    const int y = x + k * Nt;
+    // tile t covers y iff t*T in [y - T - m + 1, y + m - 1]
+    const int lo = y - T - m1, hi = y + m1;
+    int tlo = lo >= 0 ? (lo + T - 1) / T : -((-lo) / T);  // ceil(lo / T)
+    int thi = hi >= 0 ? hi / T : -((-hi + T - 1) / T);   // floor(hi / T)
+    tlo = max(tlo, 0);
+    thi = min(thi, nt - 1);
+    for (int t = tlo; t <= thi && n < 3; ++t) {
+      const int p = y - t * T + m1;
+      if (p >= 0 && p < P) {
+        tl[n] = t;
+        pl[n] = p;
+        ++n;
+      }
+    }
+  }
+  return n;
+}
+
+template <typename R, int W>
+__global__ __launch_bounds__(256) void k_fold3(KArgs a, typename cpx<R>::T* __restrict__ grid) {
+  using C = typename cpx<R>::T;
+  using G = Geo3<R, W>;
+  const C* scr = (const C*)a.scratch;
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int Nx = a.Nt[0], Ny = a.Nt[1], Nz = a.Nt[2];
+  const int m1 = a.m - 1;
+  int tx[3] = {0, 0, 0}, px[3] = {0, 0, 0};
+  const int nx = x < Nx ? cover3<G::T, G::P>(x, Nx, a.ntiles[0], m1, tx, px) : 0;
+  for (int row = blockIdx.y; row < Ny * Nz; row += gridDim.y) {
+    const int y = row % Ny, z = row / Ny;
+    int ty[3] = {0, 0, 0}, py[3] = {0, 0, 0}, tz[3] = {0, 0, 0}, pz[3] = {0, 0, 0};
+    const int ny = cover3<G::T, G::P>(y, Ny, a.ntiles[1], m1, ty, py);
+    const int nz = cover3<G::T, G::P>(z, Nz, a.ntiles[2], m1, tz, pz);
+    if (x >= Nx) continue;
+    R sr = 0, si = 0;
+#pragma unroll
+    for (int c2 = 0; c2 < 3; ++c2) {
+      if (c2 >= nz) break;
+#pragma unroll
+      for (int c1 = 0; c1 < 3; ++c1) {
+        if (c1 >= ny) break;
+        const int trow = a.ntiles[0] * (ty[c1] + a.ntiles[1] * tz[c2]);
+        const int orow = G::P * (py[c1] + G::P * pz[c2]);
+#pragma unroll
+        for (int c0 = 0; c0 < 3; ++c0) {
+          if (c0 >= nx) break;
+          const int t = tx[c0] + trow;
+          const int off = px[c0] + orow;
+          const int e0 = a.tstart[t + 1];
+          for (int itm = a.tstart[t]; itm < e0; ++itm) {
+            const C v = scr[(size_t)itm * G::PAD + off];
+            sr += v.x;
+            si += v.y;
+          }
+        }
+      }
+    }
+    grid[(long long)row * Nx + x] = C{sr, si};
+  }
+}
+
+template <typename R, int W>
+static hipError_t go3(bool spread, int NT, const KArgs& a, hipStream_t s, const void* in, void* out) {
+  using C = typename cpx<R>::T;
+  using G = Geo3<R, W>;
+  const int nblk = 8 * ((a.nitems + 7) / 8);
+  const int lds = (int)G::LDS;
+  if (spread) {
+    auto k = NT == 1024 ? k_spread3<R, W, 1024> : k_spread3<R, W, 512>;
+    if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
+      return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(NT), lds, s, a, (const C*)in);
+    (void)out;
+  } else {
+    auto k = NT == 1024 ? k_interp3<R, W, 1024> : k_interp3<R, W, 512>;
+    if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
+      return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(NT), lds, s, a, (const C*)in, (C*)out);
+  }
+  return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t go3w(bool spread, int W, int NT, const KArgs& a, hipStream_t s, const void* in, void* out) {
+  switch (W) {
+    case 2: return go3<R, 2>(spread, NT, a, s, in, out);
+    case 4: return go3<R, 4>(spread, NT, a, s, in, out);
+    case 6: return go3<R, 6>(spread, NT, a, s, in, out);
+    case 8: return go3<R, 8>(spread, NT, a, s, in, out);
+    case 10: return go3<R, 10>(spread, NT, a, s, in, out);
+    case 12: return go3<R, 12>(spread, NT, a, s, in, out);
+    case 14: return go3<R, 14>(spread, NT, a, s, in, out);
+    case 16: return go3<R, 16>(spread, NT, a, s, in, out);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename R, int W>
+static hipError_t fold3(const KArgs& a, hipStream_t s, void* grid) {
+  using C = typename cpx<R>::T;
+  const long long rows = (long long)a.Nt[1] * a.Nt[2];
+  hipLaunchKernelGGL((k_fold3<R, W>), dim3((a.Nt[0] + 255) / 256, (unsigned)std::min<long long>(rows, 65535)),
+                     dim3(256), 0, s, a, (C*)grid);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_d3(bool dbl, int W, const KArgs& a, hipStream_t s, void* grid) {
+#define NFFTM_F3(WW) \
+  case WW:           \
+    return dbl ? fold3<double, WW>(a, s, grid) : fold3<float, WW>(a, s, grid);
+  switch (W) {
+    NFFTM_F3(2) NFFTM_F3(4) NFFTM_F3(6) NFFTM_F3(8) NFFTM_F3(10) NFFTM_F3(12) NFFTM_F3(14) NFFTM_F3(16)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef NFFTM_F3
+}
+
+// 3-D tile kernels (spreading writes per-item padded tiles to a.scratch; the
+// fold is launched separately by launch_fold).
+hipError_t launch_tile_d3(bool spread, bool dbl, int W, int NT, const KArgs& a, size_t, hipStream_t s,
+                          const void* in, void* out) {
+  return dbl ? go3w<double>(spread, W, NT, a, s, in, out) : go3w<float>(spread, W, NT, a, s, in, out);
+}
+
+template <typename R>
+static void geo3(int W, int* T, size_t* lds, int* P) {
+#define NFFTM_G(WW)               \
+  case WW:                        \
+    *T = Geo3<R, WW>::T;          \
+    *P = Geo3<R, WW>::P;          \
+    *lds = Geo3<R, WW>::LDS;      \
+    break;
+  switch (W) {
+    NFFTM_G(2) NFFTM_G(4) NFFTM_G(6) NFFTM_G(8) NFFTM_G(10) NFFTM_G(12) NFFTM_G(14) NFFTM_G(16)
+    default:
+      *T = 0;
+  }
+#undef NFFTM_G
+}
+
+void tile3_geometry(bool dbl, int W, int* T, size_t* lds, int* P) {
+  if (dbl) geo3<double>(W, T, lds, P);
+  else geo3<float>(W, T, lds, P);
+}
+
 }  // namespace nfftm

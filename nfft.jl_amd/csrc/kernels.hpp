@@ -190,6 +190,18 @@ __device__ __forceinline__ double dpp(double v) {
   u.y = __builtin_amdgcn_update_dpp(0, u.y, CTRL, 0xF, 0xF, false);
   return *reinterpret_cast<double*>(&u);
 }
+// DPP move restricted to the rows of ROWMASK (other lanes read 0).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_rows(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_rows(double v) {
+  int2 u = *reinterpret_cast<int2*>(&v);
+  u.x = __builtin_amdgcn_update_dpp(0, u.x, CTRL, ROWMASK, 0xF, false);
+  u.y = __builtin_amdgcn_update_dpp(0, u.y, CTRL, ROWMASK, 0xF, false);
+  return *reinterpret_cast<double*>(&u);
+}
 __device__ __forceinline__ float readlane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -369,7 +381,7 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
     __syncthreads();
     R B = 0;
     for (int w = 0; w < NW; ++w) B += red[w];
-    B = (R)((double)B * a.wmaxD * 1.0000001);
+    B = (R)((double)B * a.oscale * a.wmaxD * 1.0000001);
     int e = 0;
     if (B > 0) (void)frexp((double)B, &e);  // B < 2^e
     fscale = (R)ldexp(1.0, 30 - e);

@@ -38,6 +38,11 @@ hipError_t launch_deconv_t(bool dbl, const DArgs& a, hipStream_t s, const void* 
 hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s);
 
 // Sum the per-item padded tiles in a.scratch into the grid (D <= 3).
-hipError_t launch_fold(int D, bool dbl, const KArgs& a, hipStream_t s, void* grid);
+hipError_t launch_fold(int D, bool dbl, int W, const KArgs& a, hipStream_t s, void* grid);
+hipError_t launch_fold_d3(bool dbl, int W, const KArgs& a, hipStream_t s, void* grid);
+
+// Compile-time tile geometry of the 3-D kernels: tile edge T, padded edge P,
+// dynamic LDS bytes.
+void tile3_geometry(bool dbl, int W, int* T, size_t* lds, int* P);
 
 }  // namespace nfftm

@@ -73,6 +73,34 @@ double kb_psi_hat(double n, double Nt, int m, double b) {
   return i0_series(m * std::sqrt(b * b - t * t));
 }
 
+// Chebyshev interpolant with n first-kind nodes on [a, b]
+// (BasicInterpolators.ChebyshevInterpolator, used at src/precomputation.jl:351):
+// coefficients from the discrete cosine sums, evaluation by Clenshaw.
+struct ChebInterp {
+  double a, b;
+  std::vector<double> c;
+  template <class F>
+  ChebInterp(F f, double a_, double b_, int n) : a(a_), b(b_), c(n, 0.0) {
+    std::vector<double> v(n);
+    for (int i = 0; i < n; ++i) v[i] = f(0.5 * (b + a) + 0.5 * (b - a) * std::cos(M_PI * (i + 0.5) / n));
+    for (int j = 0; j < n; ++j) {
+      double s = 0;
+      for (int i = 0; i < n; ++i) s += v[i] * std::cos(M_PI * j * (i + 0.5) / n);
+      c[j] = (j == 0 ? 1.0 : 2.0) * s / n;
+    }
+  }
+  double operator()(double t) const {
+    const double x = (2.0 * t - a - b) / (b - a);
+    double b1 = 0, b2 = 0;
+    for (int j = (int)c.size() - 1; j >= 1; --j) {
+      const double b0 = 2.0 * x * b1 - b2 + c[j];
+      b2 = b1;
+      b1 = b0;
+    }
+    return x * b1 - b2 + c[0];
+  }
+};
+
 // LUTSize = 2^K * m with K from NFFT3's table (src/precomputation.jl:14-17).
 i64 lut_size(int m) {
   static const int K[9] = {1, 3, 7, 9, 14, 17, 20, 23, 24};
@@ -472,9 +500,22 @@ void choose_tiles(nfftm_plan_s& p) {
     return bytes <= budget;
   };
   if (D == 3) {
-    p.NT = 512;
-    for (int t : {16, 12, 10, 8, 6, 4})
-      if (fit(t, t, t)) return;
+    // compile-time geometry of k_tile3.hip; grids smaller than a padded tile
+    // fail tiles_ok and take the node-parallel path
+    p.NT = 1024;
+    int T = 0, P = 0;
+    size_t lds = 0;
+    tile3_geometry(dbl, W, &T, &lds, &P);
+    for (int d = 0; d < 3; ++d) {
+      p.T[d] = T;
+      p.Pd[d] = P;
+      p.ntiles[d] = (int)((p.Nt[d] + T - 1) / T);
+    }
+    p.Sx = P;
+    p.Sxy = P * P;
+    p.tileElems = P * P * P;
+    p.lds = lds;
+    return;
   } else if (D == 2) {
     p.NT = 256;
     int t = 64;
@@ -625,15 +666,24 @@ int build_tables(nfftm_plan_s& p) {
     else upload_table<float>(p.d_poly, p.poly, p.wscale, e);
   }
   HIPCHK(e);
-  // deconvolution factors 1/psi_hat(n), n = i - floor(N/2)
-  // (src/precomputation.jl:345-358), FP64, times the inverse window scale
+  // deconvolution factors (src/precomputation.jl:345-358): 1/cheb(j),
+  // j = 1..N_d, cheb the 30-node Chebyshev interpolant on [1, N_d] of
+  // kappa(j) = psi_hat(j - 1 - floor(N_d/2)); FP64, times the inverse window
+  // scale.  (At m = 6, sigma = 1.25 the interpolant differs from psi_hat by
+  // ~1e-11, so it is built exactly as the reference builds it.)
   p.deconv.assign(p.D, {});
   std::vector<double> flat;
   for (int d = 0; d < p.D; ++d) {
     p.deconv[d].resize(p.N[d]);
     const i64 Na = p.N[d] / 2;
-    for (i64 i = 0; i < p.N[d]; ++i)
-      p.deconv[d][i] = 1.0 / kb_psi_hat((double)(i - Na), (double)p.Nt[d], m, p.b) / p.wscale;
+    const double Ntd = (double)p.Nt[d];
+    auto kappa = [&](double j) { return kb_psi_hat(j - 1.0 - (double)Na, Ntd, m, p.b); };
+    if (p.N[d] > 1) {
+      ChebInterp cheb(kappa, 1.0, (double)p.N[d], 30);
+      for (i64 i = 0; i < p.N[d]; ++i) p.deconv[d][i] = 1.0 / cheb((double)(i + 1)) / p.wscale;
+    } else {
+      p.deconv[d][0] = 1.0 / kappa(1.0) / p.wscale;
+    }
     flat.insert(flat.end(), p.deconv[d].begin(), p.deconv[d].end());
   }
   HIPCHK(p.d_deconv.alloc(flat.size()));
@@ -672,7 +722,7 @@ int run_conv(nfftm_plan_s& p, bool spread, const void* in, void* out, bool stand
   HIPCHK(e);
   if (spread && p.tiled) {
     ProfScope ps(p, 3);
-    HIPCHK(launch_fold(p.D, p.dbl(), a, p.stream, out));
+    HIPCHK(launch_fold(p.D, p.dbl(), p.W, a, p.stream, out));
   }
   return NFFTM_OK;
 }
