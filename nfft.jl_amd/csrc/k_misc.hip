@@ -10,44 +10,60 @@ namespace nfftm {
 // fill!(tmpVec, 0) (src/implementation.jl:159) followed by deconvolve!
 // (src/deconvolution.jl:22-45): image index i_d < Na_d goes to grid index
 // i_d - Na_d + Nt_d, the rest to i_d - Na_d.  The factor product is FP64 in
-// both precisions.
+// both precisions.  One thread per pair of grid points along dimension 1
+// (Ntilde is even), grid-stride, divisions by multiply-high.
+__device__ __forceinline__ int img_index(int g, int N, int Nt, int Na) {
+  if (g < N - Na) return g + Na;
+  if (g >= Nt - Na) return g - (Nt - Na);
+  return -1;
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_deconv(DArgs a, const typename cpx<R>::T* __restrict__ f,
                                                 typename cpx<R>::T* __restrict__ g) {
   using C = typename cpx<R>::T;
-  const int x = blockIdx.x * 256 + threadIdx.x;
-  const int nrows = (int)(a.Q / a.Nt[0]);
-  const int Nt0 = a.Nt[0], Na0 = a.Na[0], N0 = a.N[0];
-  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
-    // outer dims of this row (uniform over the block)
+  const int Nt0 = a.Nt[0], N0 = a.N[0], Na0 = a.Na[0];
+  const unsigned half = (unsigned)(Nt0 >> 1);
+  const long long npairs = a.Q >> 1;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < npairs; i += (long long)gridDim.x * 256) {
+    const unsigned row = fdiv((unsigned)(i >> 0), a.div_half);  // i < 2^31 (checked on the host)
+    const int x = 2 * (int)(i - (long long)row * half);
     bool inside = true;
     long long fo = 0;
-    double facr = 1.0;
-    int r = row;
+    double facr = a.oscale;
+    unsigned r = row;
     for (int d = 1; d < a.D; ++d) {
-      const int gd = r % a.Nt[d];
-      r /= a.Nt[d];
-      int id;
-      if (gd < a.N[d] - a.Na[d]) id = gd + a.Na[d];
-      else if (gd >= a.Nt[d] - a.Na[d]) id = gd - (a.Nt[d] - a.Na[d]);
-      else { inside = false; id = 0; }
+      const unsigned q = fdiv(r, a.div_nt[d]);
+      const int gd = (int)(r - q * (unsigned)a.Nt[d]);
+      r = q;
+      const int id = img_index(gd, a.N[d], a.Nt[d], a.Na[d]);
+      if (id < 0) {
+        inside = false;
+        break;
+      }
       fo += (long long)id * a.nstr[d];
       facr *= a.fac[d][id];
     }
-    if (x >= Nt0) continue;
-    C out;
-    out.x = 0;
-    out.y = 0;
-    int i0 = -1;
-    if (x < N0 - Na0) i0 = x + Na0;
-    else if (x >= Nt0 - Na0) i0 = x - (Nt0 - Na0);
-    if (inside && i0 >= 0) {
-      const C v = f[fo + i0];
-      const double s = a.fac[0][i0] * facr * a.oscale;
-      out.x = (R)((double)v.x * s);
-      out.y = (R)((double)v.y * s);
+    C o0, o1;
+    o0.x = o0.y = o1.x = o1.y = 0;
+    if (inside) {
+      const int i0 = img_index(x, N0, Nt0, Na0), i1 = img_index(x + 1, N0, Nt0, Na0);
+      if (i0 >= 0) {
+        const C v = f[fo + i0];
+        const double s = a.fac[0][i0] * facr;
+        o0.x = (R)((double)v.x * s);
+        o0.y = (R)((double)v.y * s);
+      }
+      if (i1 >= 0) {
+        const C v = f[fo + i1];
+        const double s = a.fac[0][i1] * facr;
+        o1.x = (R)((double)v.x * s);
+        o1.y = (R)((double)v.y * s);
+      }
     }
-    g[(long long)row * Nt0 + x] = out;
+    C* dst = g + (long long)row * Nt0 + x;
+    dst[0] = o0;
+    dst[1] = o1;
   }
 }
 
@@ -193,8 +209,8 @@ hipError_t launch_generic(bool spread, bool dbl, int W, const KArgs& a, hipStrea
 }
 
 hipError_t launch_deconv(bool dbl, const DArgs& a, hipStream_t s, const void* f, void* g) {
-  const long long rows = a.Q / a.Nt[0];
-  const dim3 grid((a.Nt[0] + 255) / 256, (unsigned)std::min<long long>(rows, 65535)), blk(256);
+  const long long blocks = std::min<long long>((a.Q / 2 + 255) / 256, 8192);
+  const dim3 grid((unsigned)blocks), blk(256);
   if (dbl)
     hipLaunchKernelGGL(k_deconv<double>, grid, blk, 0, s, a, (const double2*)f, (double2*)g);
   else

@@ -269,26 +269,38 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
   }
 }
 
-// Fold for D = 3 (see k_fold in kernels.hpp); T, P compile-time.
+// Fold for D = 3, one workgroup per output tile O (origin O_d, extent
+// E_d <= T): thread (x, y) owns the column O + (x, y, 0..E_z-1) in registers.
+// Tile t covers the arc [tT - m + 1, tT - m + 1 + P) of the periodic axis; its
+// intersection with [O_d, O_d + E_d) gives up to two (t, lo, hi, off) pieces
+// per candidate t in {o-2 .. o+2} (deduplicated), with padded offset
+// p = x - off.  For every combination of pieces, every work item of that
+// tile adds its scratch slot over the intersection box.
+struct Piece {
+  int t, lo, hi, off;  // cells [lo, hi) of the axis, padded offset x - off
+};
+
 template <int T, int P>
-__device__ __forceinline__ int cover3(int x, int Nt, int nt, int m1, int* tl, int* pl) {
+__device__ __forceinline__ int pieces(int o, int Nt, int nt, int m1, Piece* pc) {
+  const int O = o * T, E = min(T, Nt - O);
   int n = 0;
-#pragma unroll
-  for (int k = -1; k <= 1; ++k) {
-    const int y = x + k * Nt;
-    // tile t covers y iff t*T in [y - T - m + 1, y + m - 1]
-    const int lo = y - T - m1, hi = y + m1;
-    int tlo = lo >= 0 ? (lo + T - 1) / T : -((-lo) / T);  // ceil(lo / T)
-    int thi = hi >= 0 ? hi / T : -((-hi + T - 1) / T);   // floor(hi / T)
-    tlo = max(tlo, 0);
-    thi = min(thi, nt - 1);
-    for (int t = tlo; t <= thi && n < 3; ++t) {
-      const int p = y - t * T + m1;
-      if (p >= 0 && p < P) {
-        tl[n] = t;
-        pl[n] = p;
-        ++n;
-      }
+  int seen[5];
+  int ns = 0;
+  for (int dt = -2; dt <= 2; ++dt) {
+    int t = o + dt;
+    t = ((t % nt) + nt) % nt;
+    bool dup = false;
+    for (int q = 0; q < ns; ++q) dup |= seen[q] == t;
+    if (dup) continue;
+    seen[ns++] = t;
+    int start = t * T - m1;
+    start = ((start % Nt) + Nt) % Nt;
+    const int a0[2] = {start, 0};
+    const int b0[2] = {min(start + P, Nt), start + P > Nt ? start + P - Nt : 0};
+    const int offs[2] = {start, start - Nt};
+    for (int k = 0; k < 2; ++k) {
+      const int lo = max(a0[k], O), hi = min(b0[k], O + E);
+      if (lo < hi && n < 10) pc[n++] = Piece{t, lo, hi, offs[k]};
     }
   }
   return n;
@@ -298,42 +310,58 @@ template <typename R, int W>
 __global__ __launch_bounds__(256) void k_fold3(KArgs a, typename cpx<R>::T* __restrict__ grid) {
   using C = typename cpx<R>::T;
   using G = Geo3<R, W>;
+  constexpr int T = G::T, P = G::P;
   const C* scr = (const C*)a.scratch;
-  const int x = blockIdx.x * 256 + threadIdx.x;
-  const int Nx = a.Nt[0], Ny = a.Nt[1], Nz = a.Nt[2];
   const int m1 = a.m - 1;
-  int tx[3] = {0, 0, 0}, px[3] = {0, 0, 0};
-  const int nx = x < Nx ? cover3<G::T, G::P>(x, Nx, a.ntiles[0], m1, tx, px) : 0;
-  for (int row = blockIdx.y; row < Ny * Nz; row += gridDim.y) {
-    const int y = row % Ny, z = row / Ny;
-    int ty[3] = {0, 0, 0}, py[3] = {0, 0, 0}, tz[3] = {0, 0, 0}, pz[3] = {0, 0, 0};
-    const int ny = cover3<G::T, G::P>(y, Ny, a.ntiles[1], m1, ty, py);
-    const int nz = cover3<G::T, G::P>(z, Nz, a.ntiles[2], m1, tz, pz);
-    if (x >= Nx) continue;
-    R sr = 0, si = 0;
+  const int tile = blockIdx.x;
+  const int ox = tile % a.ntiles[0], oq = tile / a.ntiles[0];
+  const int oy = oq % a.ntiles[1], oz = oq / a.ntiles[1];
+  const int Nx = a.Nt[0], Ny = a.Nt[1], Nz = a.Nt[2];
+  __shared__ Piece px[10], py[10], pz[10];
+  __shared__ int np[3];
+  if (threadIdx.x == 0) {
+    np[0] = pieces<T, P>(ox, Nx, a.ntiles[0], m1, px);
+    np[1] = pieces<T, P>(oy, Ny, a.ntiles[1], m1, py);
+    np[2] = pieces<T, P>(oz, Nz, a.ntiles[2], m1, pz);
+  }
+  __syncthreads();
+  const int npx = np[0], npy = np[1], npz = np[2];
+  const int lx = threadIdx.x % T, ly = threadIdx.x / T;
+  const int x = ox * T + lx, y = oy * T + ly, z0 = oz * T;
+  const bool mine = threadIdx.x < T * T && x < Nx && y < Ny;
+  C acc[T];
 #pragma unroll
-    for (int c2 = 0; c2 < 3; ++c2) {
-      if (c2 >= nz) break;
+  for (int k = 0; k < T; ++k) acc[k] = C{0, 0};
+  for (int iz = 0; iz < npz; ++iz) {
+    const Piece qz = pz[iz];
+    for (int iy = 0; iy < npy; ++iy) {
+      const Piece qy = py[iy];
+      const bool iny = y >= qy.lo && y < qy.hi;
+      for (int ix = 0; ix < npx; ++ix) {
+        const Piece qx = px[ix];
+        const bool in = mine && iny && x >= qx.lo && x < qx.hi;
+        const int t = qx.t + a.ntiles[0] * (qy.t + a.ntiles[1] * qz.t);
+        const int i0 = a.tstart[t], i1 = a.tstart[t + 1];
+        for (int itm = i0; itm < i1; ++itm) {
+          const C* s = scr + (size_t)itm * G::PAD + (x - qx.off) + P * ((y - qy.off) + P * (-qz.off));
 #pragma unroll
-      for (int c1 = 0; c1 < 3; ++c1) {
-        if (c1 >= ny) break;
-        const int trow = a.ntiles[0] * (ty[c1] + a.ntiles[1] * tz[c2]);
-        const int orow = G::P * (py[c1] + G::P * pz[c2]);
-#pragma unroll
-        for (int c0 = 0; c0 < 3; ++c0) {
-          if (c0 >= nx) break;
-          const int t = tx[c0] + trow;
-          const int off = px[c0] + orow;
-          const int e0 = a.tstart[t + 1];
-          for (int itm = a.tstart[t]; itm < e0; ++itm) {
-            const C v = scr[(size_t)itm * G::PAD + off];
-            sr += v.x;
-            si += v.y;
+          for (int k = 0; k < T; ++k) {
+            const int z = z0 + k;
+            if (z >= qz.lo && z < qz.hi && in) {
+              const C v = s[P * P * z];
+              acc[k].x += v.x;
+              acc[k].y += v.y;
+            }
           }
         }
       }
     }
-    grid[(long long)row * Nx + x] = C{sr, si};
+  }
+  if (mine) {
+    const int E = min(T, Nz - z0);
+#pragma unroll
+    for (int k = 0; k < T; ++k)
+      if (k < E) grid[((long long)(z0 + k) * Ny + y) * Nx + x] = acc[k];
   }
 }
 
@@ -376,9 +404,9 @@ static hipError_t go3w(bool spread, int W, int NT, const KArgs& a, hipStream_t s
 template <typename R, int W>
 static hipError_t fold3(const KArgs& a, hipStream_t s, void* grid) {
   using C = typename cpx<R>::T;
-  const long long rows = (long long)a.Nt[1] * a.Nt[2];
-  hipLaunchKernelGGL((k_fold3<R, W>), dim3((a.Nt[0] + 255) / 256, (unsigned)std::min<long long>(rows, 65535)),
-                     dim3(256), 0, s, a, (C*)grid);
+  constexpr int T = Geo3<R, W>::T;
+  static_assert(T * T <= 256, "fold3 maps one thread per tile column");
+  hipLaunchKernelGGL((k_fold3<R, W>), dim3((unsigned)a.ntile_all), dim3(256), 0, s, a, (C*)grid);
   return hipGetLastError();
 }
 

@@ -14,6 +14,8 @@ struct DArgs {
   const double* fac[MAXD];  // 1/psi_hat per dim, FP64, N[d] entries
   long long P, Q;
   double oscale;  // 1 inside a transform; wscale^D for standalone deconvolve!
+  FastDiv div_half;        // Nt[0] / 2
+  FastDiv div_nt[MAXD];    // Nt[d]
 };
 
 // LDS bytes of a tile kernel.

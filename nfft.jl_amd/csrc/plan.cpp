@@ -437,6 +437,8 @@ struct nfftm_plan_s {
     a.P = P;
     a.Q = Q;
     a.oscale = 1.0;
+    a.div_half = make_fastdiv((uint32_t)std::max<i64>(1, Nt[0] / 2));
+    for (int d = 0; d < D; ++d) a.div_nt[d] = make_fastdiv((uint32_t)Nt[d]);
     return a;
   }
 };
@@ -772,6 +774,8 @@ int create_impl(nfftm_plan_s& p, const double* k) {
   for (int d = 0; d < p.D; ++d)
     if (p.Nt[d] > 0x7fffffff)
       return fail(NFFTM_ERR_UNSUPPORTED, "oversampled size %lld exceeds 2^31-1", (long long)p.Nt[d]);
+  if (p.Q / 2 > 0x7fffffffLL)
+    return fail(NFFTM_ERR_UNSUPPORTED, "oversampled grid of %lld points exceeds 2^32", (long long)p.Q);
   p.tiled = (p.flags & NFFTM_FLAG_BLOCKING) && p.D <= 3;
   if (p.tiled) {
     choose_tiles(p);
