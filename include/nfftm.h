@@ -83,6 +83,19 @@ int nfftm_forward(nfftm_plan p, const void* f_dev, void* fhat_dev);
 /* mul!(f, adjoint(p), fHat): src/implementation.jl:176-193 (interface.jl:178). */
 int nfftm_adjoint(nfftm_plan p, const void* fhat_dev, void* f_dev);
 
+/* Node gradient of the forward transform fHat = A(k) f contracted with a
+ * cotangent ybar:  kbar[d + D*j] = Re(conj(ybar_j) * d fHat_j / d k_{d,j}).
+ * Replaces the NoTangent() that the reference's rrule returns for the nodes
+ * (AbstractNFFTs/ext/AbstractNFFTsChainRulesCoreExt.jl:14-22, 26-45); the
+ * pullback of y = A(k) f is then (kbar, adjoint(p) * ybar).  Computed as the
+ * forward path (deconvolve!, forward FFT) followed by a gather with the
+ * analytic derivative of the Kaiser-Bessel window (of the fitted polynomial
+ * in POLYNOMIAL/TENSOR mode), window arithmetic in FP64.
+ * f: prod(N) complex, ybar: M complex, kbar: D*M doubles laid out as k
+ * (column-major D x M, caller's node order); all device pointers.  Uses the
+ * plan's grid. */
+int nfftm_grad_nodes(nfftm_plan p, const void* f_dev, const void* ybar_dev, double* kbar_dev);
+
 /* deconvolve!(p, f, g): src/deconvolution.jl:2-45, including the
  * fill!(tmpVec, 0) of src/implementation.jl:159.  g: prod(Ntilde) complex. */
 int nfftm_deconvolve(nfftm_plan p, const void* f_dev, void* g_dev);

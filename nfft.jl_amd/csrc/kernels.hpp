@@ -91,6 +91,7 @@ struct KArgs {
   const void* lut;  // R[LUTSize + 2]
   long long scale;  // LUTSize / m, a power of two
   const void* poly; // R[(2m+1) * 2m], column l holds coefficients of tap l
+  const double* polyd;  // the same coefficients in FP64 (node gradient)
   double b;         // Kaiser-Bessel shape pi*(2 - 1/sigma)
   double wscale;    // power-of-two window normalisation
   double oscale;    // output scale: 1 inside a transform; wscale^-D for the

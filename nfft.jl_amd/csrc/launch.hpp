@@ -35,6 +35,11 @@ hipError_t launch_tile_d3(bool spread, bool dbl, int W, int NT, const KArgs& a, 
 hipError_t launch_generic(bool spread, bool dbl, int W, const KArgs& a, hipStream_t s,
                           const void* in, void* out);
 
+// Node gradient kbar[d + D j] = Re(conj(ybar_j) d fhat_j / d k_{d,j}) from
+// the transformed grid g of the forward path.
+hipError_t launch_grad_nodes(bool dbl, int W, const KArgs& a, hipStream_t s, const void* g,
+                             const void* ybar, double* kbar);
+
 hipError_t launch_deconv(bool dbl, const DArgs& a, hipStream_t s, const void* f, void* g);
 hipError_t launch_deconv_t(bool dbl, const DArgs& a, hipStream_t s, const void* g, void* f);
 hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s);

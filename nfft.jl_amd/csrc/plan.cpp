@@ -309,7 +309,7 @@ struct nfftm_plan_s {
   std::vector<double> lut, poly;
   std::vector<std::vector<double>> deconv;
   DevBuf<char> d_lut, d_poly, d_grid;
-  DevBuf<double> d_deconv, d_ks;
+  DevBuf<double> d_deconv, d_ks, d_polyd;
   DevBuf<int> d_perm;
   DevBuf<int4> d_items;
   DevBuf<int> d_tstart;
@@ -404,6 +404,7 @@ struct nfftm_plan_s {
     a.lut = d_lut.p;
     a.scale = LUTSize / m;
     a.poly = d_poly.p;
+    a.polyd = d_polyd.p;
     a.b = b;
     a.wscale = wscale;
     a.oscale = 1.0;
@@ -666,6 +667,13 @@ int build_tables(nfftm_plan_s& p) {
     p.poly = poly_fit(m, p.b);
     if (p.dbl()) upload_table<double>(p.d_poly, p.poly, p.wscale, e);
     else upload_table<float>(p.d_poly, p.poly, p.wscale, e);
+    if (e == hipSuccess) {
+      std::vector<double> pd(p.poly.size());
+      for (size_t i = 0; i < pd.size(); ++i) pd[i] = p.poly[i] * p.wscale;
+      e = p.d_polyd.alloc(pd.size());
+      if (e == hipSuccess)
+        e = hipMemcpy(p.d_polyd.p, pd.data(), pd.size() * sizeof(double), hipMemcpyHostToDevice);
+    }
   }
   HIPCHK(e);
   // deconvolution factors (src/precomputation.jl:345-358): 1/cheb(j),
@@ -883,6 +891,22 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
   if (int rc = run_deconv(*p, false, p->d_grid.p, f)) return rc;
   if (int rc = t.mark(3)) return rc;
   return t.finish(&p->timing[4], &p->timing[5], &p->timing[6]);
+}
+
+int nfftm_grad_nodes(nfftm_plan p, const void* f, const void* ybar, double* kbar) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if ((!f && p->P) || (!ybar && p->M) || (!kbar && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
+  if (int rc = run_deconv(*p, true, f, p->d_grid.p)) return rc;
+  {
+    ProfScope ps(*p, 1);
+    if (int rc = p->fft->exec(true, p->d_grid.p, p->celem(), p->stream)) return rc;
+  }
+  if (p->M) {
+    ProfScope ps(*p, 7);
+    HIPCHK(launch_grad_nodes(p->dbl(), p->W, p->kargs(), p->stream, p->d_grid.p, ybar, kbar));
+  }
+  return NFFTM_OK;
 }
 
 int nfftm_deconvolve(nfftm_plan p, const void* f, void* g) {

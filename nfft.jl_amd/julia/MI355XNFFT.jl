@@ -1,0 +1,214 @@
+"""
+    MI355XNFFT
+
+Julia binding of libnfftm.so: an `AbstractNFFTPlan` whose transforms run on
+one MI355X.  It plugs into AbstractNFFTs exactly where NFFT.jl's own GPU
+extension does (ext/NFFTGPUArraysExt/implementation.jl:21-59) and exposes the
+same operator surface: plan_nfft, mul!, adjoint, nodes!, size_in/size_out,
+convolve!/convolve_transpose!/deconvolve!/deconvolve_transpose!, copy and
+TimingStats.  Data buffers are AMDGPU `ROCArray`s passed as device pointers;
+node matrices stay on the host, as `k::Matrix` does in the reference.
+
+Usage:
+
+    using AMDGPU, AbstractNFFTs, MI355XNFFT
+    MI355XNFFT.activate!()
+    p = plan_nfft(ROCArray, k, N; m=4, σ=2.0, precompute=AbstractNFFTs.LINEAR)
+    fHat = p * ROCArray(f)
+
+This file is written against the C ABI in include/nfftm.h.  No Julia
+toolchain exists where the library was built, so it has not been executed;
+the same calls are exercised from Python ctypes by the test suite.
+"""
+module MI355XNFFT
+
+using AbstractNFFTs
+using AbstractNFFTs: AbstractNFFTBackend, TimingStats
+using LinearAlgebra
+using AMDGPU
+import ChainRulesCore
+
+const libnfftm = get(ENV, "NFFTM_LIB", joinpath(@__DIR__, "..", "nfftm", "libnfftm.so"))
+
+const NFFTM_OK = Cint(0)
+const NFFTM_ERR_ARG = Cint(1)
+const NFFTM_ERR_DIM = Cint(2)
+const NFFTM_ERR_UNSUPPORTED = Cint(3)
+const NFFTM_F32 = Cint(0)
+const NFFTM_F64 = Cint(1)
+const NFFTM_FLAG_BLOCKING = Cint(1)
+const NFFTM_FLAG_TIMING = Cint(2)
+
+# Error classes as the reference raises them (src/utils.jl:46-55, :98-105).
+function check(rc::Cint)
+  rc == NFFTM_OK && return nothing
+  msg = unsafe_string(ccall((:nfftm_last_error, libnfftm), Cstring, ()))
+  rc == NFFTM_ERR_ARG && throw(ArgumentError(msg))
+  rc == NFFTM_ERR_DIM && throw(DimensionMismatch(msg))
+  error(msg)
+end
+
+struct MI355XBackend <: AbstractNFFTBackend end
+activate!() = AbstractNFFTs.set_active_backend!(MI355XNFFT)
+backend() = MI355XBackend()
+
+mutable struct MI355XNFFTPlan{T,D} <: AbstractNFFTs.AbstractNFFTPlan{T,D,1}
+  handle::Ptr{Cvoid}
+  N::NTuple{D,Int64}
+  NOut::NTuple{1,Int64}
+  J::Int64
+  k::Matrix{T}
+  Ñ::NTuple{D,Int64}
+  tmpVec::ROCArray{Complex{T},D}    # wraps the plan-owned grid (no copy)
+end
+
+precision_of(::Type{Float32}) = NFFTM_F32
+precision_of(::Type{Float64}) = NFFTM_F64
+
+window_id(w::Symbol) = w == :kaiser_bessel ? Cint(0) :
+  error("Window $w is not available in the MI355X backend")
+
+function wrap_grid(h::Ptr{Cvoid}, ::Type{T}, Ñ::NTuple{D,Int}) where {T,D}
+  ptr = Ref{Ptr{Cvoid}}()
+  check(ccall((:nfftm_grid, libnfftm), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), h, ptr))
+  return unsafe_wrap(ROCArray, Ptr{Complex{T}}(ptr[]), Ñ; own=false)
+end
+
+function sizes(h::Ptr{Cvoid}, D)
+  Ñ = zeros(Int64, D); M = Ref{Int64}(); σ = Ref{Float64}()
+  check(ccall((:nfftm_sizes, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Int64}, Ref{Int64}, Ref{Float64}),
+              h, Ñ, M, σ))
+  return NTuple{D,Int64}(Ñ), M[]
+end
+
+# plan_nfft(::Backend, ::Type{<:ROCArray}, k, N): ext/NFFTGPUArraysExt/implementation.jl:21-59,
+# parameters resolved as AbstractNFFTs/src/misc.jl:66-81.
+function AbstractNFFTs.plan_nfft(::MI355XBackend, ::Type{<:ROCArray}, k::Matrix{T}, N::NTuple{D,Int};
+    dims::Union{Integer,UnitRange{Int64}}=1:D, fftflags=nothing, sortNodes=false,
+    blocking=true, window=:kaiser_bessel, precompute=AbstractNFFTs.POLYNOMIAL,
+    timing::Union{Nothing,TimingStats}=nothing, kargs...) where {T<:Union{Float32,Float64},D}
+  dims != 1:D && error("MI355X NFFT does not work along directions right now!")
+  reltol, m, σ = AbstractNFFTs.accuracyParams(; kargs...)
+  sortNodes && (k = sortslices(k, dims=2))
+  kd = Matrix{Float64}(k)
+  h = Ref{Ptr{Cvoid}}(C_NULL)
+  flags = (blocking ? NFFTM_FLAG_BLOCKING : Cint(0)) | (timing === nothing ? Cint(0) : NFFTM_FLAG_TIMING)
+  t = @elapsed check(ccall((:nfftm_plan_create, libnfftm), Cint,
+      (Ref{Ptr{Cvoid}}, Cint, Cint, Ptr{Int64}, Int64, Ptr{Float64}, Cint, Cdouble, Cint, Cint, Cint, Ptr{Cvoid}),
+      h, precision_of(T), D, collect(Int64, N), size(k, 2), kd, m, σ, window_id(window),
+      Cint(Int(precompute)), flags, AMDGPU.stream().stream))
+  timing === nothing || (timing.pre = t)
+  Ñ, M = sizes(h[], D)
+  p = MI355XNFFTPlan{T,D}(h[], N, (Int64(M),), Int64(M), k, Ñ, wrap_grid(h[], T, Ñ))
+  finalizer(p) do q   # Wrappers/FINUFFT.jl:52-56
+    ccall((:nfftm_plan_destroy, libnfftm), Cvoid, (Ptr{Cvoid},), q.handle)
+  end
+  return p
+end
+
+AbstractNFFTs.size_in(p::MI355XNFFTPlan) = p.N
+AbstractNFFTs.size_out(p::MI355XNFFTPlan) = p.NOut
+
+# nodes!(p, k): src/implementation.jl:108-141
+function AbstractNFFTs.nodes!(p::MI355XNFFTPlan{T,D}, k::Matrix{T}) where {T,D}
+  check(ccall((:nfftm_nodes, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64),
+              p.handle, Matrix{Float64}(k), size(k, 2)))
+  p.k = k; p.J = size(k, 2); p.NOut = (p.J,)
+  return p
+end
+
+function Base.copy(p::MI355XNFFTPlan{T,D}) where {T,D}   # src/implementation.jl:45-66
+  h = Ref{Ptr{Cvoid}}(C_NULL)
+  check(ccall((:nfftm_plan_copy, libnfftm), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), p.handle, h))
+  q = MI355XNFFTPlan{T,D}(h[], p.N, p.NOut, p.J, copy(p.k), p.Ñ, wrap_grid(h[], T, p.Ñ))
+  finalizer(x -> ccall((:nfftm_plan_destroy, libnfftm), Cvoid, (Ptr{Cvoid},), x.handle), q)
+  return q
+end
+
+devptr(a::ROCArray) = Ptr{Cvoid}(pointer(a))
+
+# src/utils.jl:98-105
+function consistency(p::MI355XNFFTPlan{T,D}, f::ROCArray, fHat::ROCArray) where {T,D}
+  size(f) == p.N || throw(DimensionMismatch("Data f has size $(size(f)) but the plan expects $(p.N)"))
+  size(fHat, 1) == p.J || throw(DimensionMismatch("fHat has length $(size(fHat,1)) but the plan has $(p.J) nodes"))
+  (eltype(f) == Complex{T} && eltype(fHat) == Complex{T}) ||
+    throw(ArgumentError("data must be Complex{$T} for this plan"))
+end
+
+function fill_timing!(p, timing, fwd::Bool)
+  timing === nothing && return
+  t = zeros(Float64, 7)
+  check(ccall((:nfftm_timing, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Float64}), p.handle, t))
+  if fwd
+    timing.deconv, timing.fft, timing.conv = t[4], t[3], t[2]
+  else
+    timing.conv_adjoint, timing.fft_adjoint, timing.deconv_adjoint = t[5], t[6], t[7]
+  end
+end
+
+# mul!(fHat, p, f): src/implementation.jl:155-172
+function LinearAlgebra.mul!(fHat::ROCArray, p::MI355XNFFTPlan{T}, f::ROCArray;
+                            verbose=false, timing::Union{Nothing,TimingStats}=nothing) where {T}
+  consistency(p, f, fHat)
+  check(ccall((:nfftm_set_timing, libnfftm), Cint, (Ptr{Cvoid}, Cint), p.handle, timing !== nothing))
+  check(ccall((:nfftm_forward, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+              p.handle, devptr(f), devptr(fHat)))
+  fill_timing!(p, timing, true)
+  verbose && timing !== nothing && @info "Timing: deconv=$(timing.deconv) fft=$(timing.fft) conv=$(timing.conv)"
+  return fHat
+end
+
+# mul!(f, adjoint(p), fHat): src/implementation.jl:176-193
+function LinearAlgebra.mul!(f::ROCArray, pl::Adjoint{Complex{T},<:MI355XNFFTPlan{T}}, fHat::ROCArray;
+                            verbose=false, timing::Union{Nothing,TimingStats}=nothing) where {T}
+  p = pl.parent
+  consistency(p, f, fHat)
+  check(ccall((:nfftm_set_timing, libnfftm), Cint, (Ptr{Cvoid}, Cint), p.handle, timing !== nothing))
+  check(ccall((:nfftm_adjoint, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+              p.handle, devptr(fHat), devptr(f)))
+  fill_timing!(p, timing, false)
+  verbose && timing !== nothing && @info "Timing: conv=$(timing.conv_adjoint) fft=$(timing.fft_adjoint) deconv=$(timing.deconv_adjoint)"
+  return f
+end
+
+# Stage operators used by NFFTTools (samplingDensity.jl:106,112):
+# src/convolution.jl:20-53, :115-149; src/deconvolution.jl:2-92.
+for (jl, c) in ((:convolve!, :nfftm_convolve), (:convolve_transpose!, :nfftm_convolve_transpose),
+                (:deconvolve!, :nfftm_deconvolve), (:deconvolve_transpose!, :nfftm_deconvolve_transpose))
+  @eval function AbstractNFFTs.$jl(p::MI355XNFFTPlan, a::ROCArray, b::ROCArray)
+    check(ccall(($(QuoteNode(c)), libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                p.handle, devptr(a), devptr(b)))
+    return b
+  end
+end
+
+"""
+    grad_nodes(p, f, ȳ) -> k̄::Matrix{Float64}
+
+Node gradient of y = p * f contracted with the cotangent ȳ:
+k̄[d, j] = Re(conj(ȳ[j]) * ∂y[j]/∂k[d, j]), evaluated on the device with the
+analytic derivative of the Kaiser-Bessel window.
+"""
+function grad_nodes(p::MI355XNFFTPlan{T,D}, f::ROCArray, ȳ::ROCArray) where {T,D}
+  consistency(p, f, ȳ)
+  kbar = ROCArray{Float64}(undef, D, p.J)
+  check(ccall((:nfftm_grad_nodes, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+              p.handle, devptr(f), devptr(ȳ), devptr(kbar)))
+  return Array(kbar)
+end
+
+# AbstractNFFTs/ext/AbstractNFFTsChainRulesCoreExt.jl:26-45 returns NoTangent()
+# for the nodes; this rule returns their gradient as well.
+function ChainRulesCore.rrule(::typeof(AbstractNFFTs.nfft), k::Matrix{T}, x::ROCArray) where {T}
+  p = AbstractNFFTs.plan_nfft(MI355XBackend(), ROCArray, k, size(x))
+  y = p * x
+  function nfft_pullback(ȳ)
+    ȳ = ROCArray(ChainRulesCore.unthunk(ȳ))
+    k̄ = T.(grad_nodes(p, x, ȳ))
+    x̄ = adjoint(p) * ȳ
+    return ChainRulesCore.NoTangent(), k̄, x̄
+  end
+  return y, nfft_pullback
+end
+
+end # module

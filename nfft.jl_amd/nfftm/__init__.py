@@ -415,6 +415,20 @@ class NFFTPlan:
             return user
         return o
 
+    def grad_nodes(self, f, ybar):
+        """Node gradient kbar[d, j] = Re(conj(ybar_j) * d fHat_j / d k[d, j]) of
+        fHat = p * f: the node cotangent that the reference's rrule leaves as
+        NoTangent() (AbstractNFFTs/ext/AbstractNFFTsChainRulesCoreExt.jl:14-45).
+        Returns a (D, M) float64 device tensor in the axis order of k."""
+        f = self._prep(f, self.N, "f")
+        ybar = self._prep(ybar, (self.J,), "ybar")
+        kb = torch.empty((self.J, self.D), dtype=torch.float64, device=_device())
+        self._lib.nfftm_set_stream(self._handle, _stream())
+        _check(self._lib.nfftm_grad_nodes(self._handle, ctypes.c_void_p(f.data_ptr()),
+                                          ctypes.c_void_p(ybar.data_ptr()),
+                                          ctypes.c_void_p(kb.data_ptr())))
+        return kb.t().flip(0)
+
     # optional interface (AbstractNFFTs/src/interface.jl:217-243) ---------------
     def _stage(self, fn, inp, in_shape, out, out_shape, names):
         real_in = isinstance(inp, torch.Tensor) and not inp.is_complex() or (

@@ -23,7 +23,7 @@ EXPORTS = (
     "nfftm_deconvolve", "nfftm_deconvolve_transpose", "nfftm_convolve",
     "nfftm_convolve_transpose", "nfftm_fft", "nfftm_sizes", "nfftm_grid",
     "nfftm_timing", "nfftm_set_timing", "nfftm_profile", "nfftm_profile_read",
-    "nfftm_set_stream", "nfftm_plan_copy",
+    "nfftm_set_stream", "nfftm_plan_copy", "nfftm_grad_nodes",
     "nfftm_plan_destroy", "nfftm_last_error", "nfftm_version",
 )
 
@@ -48,6 +48,7 @@ def load():
     pd = ctypes.POINTER(ctypes.c_double)
     L.nfftm_plan_create.argtypes = [ctypes.POINTER(vp), i, i, pi64, i64, pd, i, d, i, i, i, vp]
     L.nfftm_nodes.argtypes = [vp, pd, i64]
+    L.nfftm_grad_nodes.argtypes = [vp, vp, vp, vp]
     for name in ("nfftm_forward", "nfftm_adjoint", "nfftm_deconvolve",
                  "nfftm_deconvolve_transpose", "nfftm_convolve", "nfftm_convolve_transpose"):
         getattr(L, name).argtypes = [vp, vp, vp]

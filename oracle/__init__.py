@@ -78,6 +78,10 @@ def lib():
             getattr(L, name).argtypes = [P, dp, dp]
         L.orc_ndft.argtypes = [I, lp, LL, dp, dp, dp, lp, LL]
         L.orc_ndft_adjoint.argtypes = [I, lp, LL, dp, dp, dp, lp, LL]
+        L.orc_window_du.restype = D
+        L.orc_window_du.argtypes = [D, I, D]
+        L.orc_grad_gather.argtypes = [P, dp, dp, dp]
+        L.orc_ndft_grad.argtypes = [I, lp, LL, dp, dp, dp, dp, lp, LL]
         _lib = L
     return _lib
 
@@ -190,6 +194,31 @@ def ndft_adjoint(k, N, fhat, sel=None):
     return out
 
 
+def ndft_grad(k, f, ybar, sel=None):
+    """Closed-form node gradient of the NDFT: kbar[d, q] =
+    Re(conj(ybar_j) * sum_n f_n (-2 pi i n_d) exp(-2 pi i n.k_j)), j = sel[q]
+    (default all nodes).  Python axis order: returns (D, len(sel))."""
+    kj, D, M = _julia_nodes(k)
+    f = _c128(f)
+    ybar = _c128(ybar)
+    N = np.array(f.shape[::-1], dtype=np.int64)
+    nout = M if sel is None else len(sel)
+    out = np.empty((nout, D))
+    if sel is None:
+        lib().orc_ndft_grad(D, _lp(N), M, _dp(kj), _dp(f.view(np.float64)),
+                            _dp(ybar.view(np.float64)), _dp(out), None, 0)
+    else:
+        sel = np.ascontiguousarray(sel, dtype=np.int64)
+        lib().orc_ndft_grad(D, _lp(N), M, _dp(kj), _dp(f.view(np.float64)),
+                            _dp(ybar.view(np.float64)), _dp(out), _lp(sel), len(sel))
+    return np.ascontiguousarray(out.T[::-1])
+
+
+def window_du(u, m, sigma):
+    """d psi / du of the Kaiser-Bessel window, u in grid units."""
+    return lib().orc_window_du(float(u), int(m), float(sigma))
+
+
 # ----------------------------------------------------------------------------
 # The plan (src/implementation.jl)
 # ----------------------------------------------------------------------------
@@ -274,6 +303,18 @@ class OraclePlan:
         g = self.convolve_transpose(fhat)
         g = self.bfft(g)
         return self.deconvolve_transpose(g)
+
+    def grad_nodes(self, f, ybar):
+        """Node gradient kbar[d, j] = Re(conj(ybar_j) d fhat_j / d k_{d,j}) of
+        fhat = forward(f), by differentiating the window of the interpolation
+        sum.  Python axis order: returns (D, M)."""
+        g = self.fft(self.deconvolve(np.asarray(f).reshape(self.N)))
+        g = _c128(g)
+        ybar = _c128(ybar)
+        out = np.empty((self.M, self.D))
+        lib().orc_grad_gather(self._p, _dp(g.view(np.float64)), _dp(ybar.view(np.float64)),
+                              _dp(out))
+        return np.ascontiguousarray(out.T[::-1])
 
 
 def relerr(a, b):

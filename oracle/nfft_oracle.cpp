@@ -72,6 +72,33 @@ double window_kb_hat(double n, double Nt, int m, double sigma) {
   return bessel_i0(m * std::sqrt(b * b - t * t));
 }
 
+// Derivative d psi / du of the Kaiser-Bessel window of src/windowFunctions.jl:
+// 21-34 as a function of u = Nt * k (grid units).  With s = sqrt(m^2 - u^2),
+// x = b s:  psi = sinh(x) / (pi s)  and
+//   d psi / du = -u (x cosh x - sinh x) / (pi s^3)
+//              = -u b^3 / pi * sum_{n>=1} x^(2n-2) 2n / (2n+1)!,
+// the series used for x < 1/2 where the closed form cancels.  Zero for
+// |u| >= m (the reference window is 0 beyond m; its value b/pi at |u| = m is
+// a single point).  Not in the reference: the node gradient is an addition.
+double window_kb_du(double u, int m, double sigma) {
+  double b = M_PI * (2.0 - 1.0 / sigma);
+  double au = std::fabs(u);
+  if (au >= (double)m) return 0.0;
+  double s = std::sqrt((double)(m * m) - u * u);
+  double x = b * s, q;
+  if (x < 0.5) {
+    double x2 = x * x, term = 1.0 / 3.0, sum = term;
+    for (int n = 1; n < 12; ++n) {
+      term *= x2 / (2.0 * n * (2.0 * n + 3.0));
+      sum += term;
+    }
+    q = b * b * b * sum;
+  } else {
+    q = (x * std::cosh(x) - std::sinh(x)) / (s * s * s);
+  }
+  return -u * q / M_PI;
+}
+
 // --------------------------------------------------------------------------
 // Parameters: src/precomputation.jl:3-56 (initParams), :59-77 (_blockSize).
 // --------------------------------------------------------------------------
@@ -364,6 +391,41 @@ void node_weights_blocked(const OPlan& p, i64 l, i64 i, int d, i64* y, double* w
     for (int q = 0; q < 2 * p.m; ++q) w[q] = t[q];
   } else {
     poly_weights(p, pr.second, w);
+  }
+}
+
+// Window values w and their node derivatives dw = d w / d k (grid index
+// arithmetic as node_weights_nonblocked) for the node gradient.  w_l depends on
+// k through u_l = k Nt - l - off with off locally constant, so
+// dw_l = Nt psi'(u_l).  POLYNOMIAL / TENSOR: w_l = P_l(x), x = k Nt - off - m
+// + 1/2 (src/precomputation.jl:215-222), dw_l = Nt P_l'(x) by Horner.  FULL and
+// LINEAR: the exact window and its analytic derivative (the LINEAR table
+// samples the exact window).
+void node_weights_grad(const OPlan& p, i64 j, int d, i64* idx, double* w, double* dw) {
+  int m = p.m, Z = 2 * m;
+  i64 Nt = p.Nt[d];
+  double kscale = p.k[d + p.D * j] * (double)Nt;
+  i64 off = (i64)std::floor(kscale) - m + 1;
+  for (int l = 0; l < Z; ++l) idx[l] = (((i64)l + off + Nt) % Nt + Nt) % Nt;
+  if (p.precompute == TENSOR || p.precompute == POLYNOMIAL) {
+    int deg = 2 * m + 1;
+    double x = (kscale - (double)off - (double)m + 0.5);
+    for (int l = 0; l < Z; ++l) {
+      const double* c = &p.poly[l * deg];
+      double s = c[deg - 1], ds = 0.0;
+      for (int i = deg - 2; i >= 0; --i) {
+        ds = std::fma(x, ds, s);
+        s = std::fma(x, s, c[i]);
+      }
+      w[l] = s;
+      dw[l] = (double)Nt * ds;
+    }
+  } else {
+    for (int l = 0; l < Z; ++l) {
+      double u = kscale - (double)l - (double)off;
+      w[l] = window_kb(u / (double)Nt, (double)Nt, m, p.sigma);
+      dw[l] = (double)Nt * window_kb_du(u, m, p.sigma);
+    }
   }
 }
 
@@ -865,6 +927,97 @@ void orc_ndft_adjoint(int D, const long long* N, long long M, const double* k, c
       s += fhat[j] * cd(std::cos(a), std::sin(a));
     }
     f[q] = s;
+  }
+}
+
+// Analytic window derivative of the window sample itself (grid units):
+// exposed so the test suite can check it against finite differences.
+double orc_window_du(double u, int m, double sigma) { return window_kb_du(u, m, sigma); }
+
+// Node gradient of fhat = A(k) f contracted with a cotangent ybar, given the
+// deconvolved and transformed grid g = FFT(deconvolve(f)) of the forward path
+// (src/implementation.jl:155-172):
+//   kbar[d + D j] = Re(conj(ybar_j) * sum_l g[idx_l] d/dk_{d,j} prod_e w_e(l_e)).
+// Differentiating the interpolation sum of src/convolution.jl:56-99 term by
+// term; the reference's rrule returns NoTangent() for the nodes
+// (AbstractNFFTs/ext/AbstractNFFTsChainRulesCoreExt.jl:14-22, 26-45).
+void orc_grad_gather(OPlan* p, const double* g_, const double* ybar_, double* kbar) {
+  const cd* g = (const cd*)g_;
+  const cd* ybar = (const cd*)ybar_;
+  int D = p->D, Z = 2 * p->m;
+#pragma omp parallel
+  {
+    std::vector<i64> idx(D * Z);
+    std::vector<double> w(D * Z), dw(D * Z);
+#pragma omp for schedule(static)
+    for (i64 j = 0; j < p->M; ++j) {
+      for (int d = 0; d < D; ++d) node_weights_grad(*p, j, d, &idx[d * Z], &w[d * Z], &dw[d * Z]);
+      cd S[MAXD];
+      for (int d = 0; d < D; ++d) S[d] = cd(0, 0);
+      int l[MAXD] = {0};
+      do {
+        i64 gi = 0, str = 1;
+        for (int d = 0; d < D; ++d) {
+          gi += idx[d * Z + l[d]] * str;
+          str *= p->Nt[d];
+        }
+        const cd v = g[gi];
+        for (int d = 0; d < D; ++d) {
+          double pw = dw[d * Z + l[d]];
+          for (int e = 0; e < D; ++e)
+            if (e != d) pw *= w[e * Z + l[e]];
+          S[d] += pw * v;
+        }
+      } while (next_multi(D, l, Z));
+      for (int d = 0; d < D; ++d) kbar[d + (i64)D * j] = (std::conj(ybar[j]) * S[d]).real();
+    }
+  }
+}
+
+// Closed-form node gradient of the direct NDFT (src/direct.jl:99-123):
+//   d fhat_j / d k_{d,j} = sum_n f_n (-2 pi i n_d) exp(-2 pi i n.k_j),
+//   kbar[d + D q] = Re(conj(ybar_j) * that), j = sel[q] (or q).
+void orc_ndft_grad(int D, const long long* N, long long M, const double* k, const double* f_,
+                   const double* ybar_, double* kbar, const long long* sel, long long nsel) {
+  const cd* f = (const cd*)f_;
+  const cd* ybar = (const cd*)ybar_;
+  i64 P = 1;
+  for (int d = 0; d < D; ++d) P *= N[d];
+  i64 nout = sel ? nsel : M;
+#pragma omp parallel
+  {
+    std::vector<std::vector<cd>> ph(D);
+    std::vector<cd> part;
+#pragma omp for schedule(dynamic, 4)
+    for (i64 q = 0; q < nout; ++q) {
+      i64 j = sel ? sel[q] : q;
+      for (int d = 0; d < D; ++d) {
+        ph[d].resize(N[d]);
+        i64 Na = N[d] / 2;
+        for (i64 i = 0; i < N[d]; ++i) {
+          double a = -2.0 * M_PI * frac_prod(k[d + (i64)D * j], (double)(i - Na));
+          ph[d][i] = cd(std::cos(a), std::sin(a));
+        }
+      }
+      for (int dd = 0; dd < D; ++dd) {  // derivative along dimension dd
+        part.assign(f, f + P);
+        i64 len = P;
+        for (int d = 0; d < D; ++d) {
+          i64 n = N[d], outer = len / n, Na = N[d] / 2;
+          for (i64 o = 0; o < outer; ++o) {
+            cd s(0, 0);
+            for (i64 i = 0; i < n; ++i) {
+              cd t = part[o * n + i] * ph[d][i];
+              if (d == dd) t *= cd(0.0, -2.0 * M_PI * (double)(i - Na));
+              s += t;
+            }
+            part[o] = s;
+          }
+          len = outer;
+        }
+        kbar[dd + (i64)D * q] = (std::conj(ybar[j]) * part[0]).real();
+      }
+    }
   }
 }
 

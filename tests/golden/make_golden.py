@@ -38,6 +38,12 @@ def psi_hat(n, Nt, m, sigma):
     return mp.besseli(0, m * mp.sqrt(b ** 2 - (2 * mp.pi * n / Nt) ** 2))
 
 
+def psi_du(u, m, sigma):
+    """d psi / du in grid units (psi(u / Nt) with Nt = 1), by mpmath's
+    numerical differentiation at 40 digits."""
+    return mp.diff(lambda x: psi(x, 1, m, sigma), mp.mpf(u))
+
+
 def ndft(k, f):
     """k: (D, M) floats, f: nested array of shape N (C order, axis d <-> k[d])."""
     f = np.asarray(f, dtype=complex)
@@ -74,7 +80,7 @@ def cplx(z):
 
 
 def main():
-    out = {"window": [], "window_hat": [], "ndft": []}
+    out = {"window": [], "window_hat": [], "ndft": [], "window_du": []}
     for (m, sigma, Nt) in [(4, 2.0, 1), (5, 2.0, 1), (6, 1.25, 1), (4, 2.0, 256), (8, 2.0, 1)]:
         for u in [0.0, 0.3, 1.0, 1.5, 2.999, 3.5, 3.999999, float(m) - 1e-9, float(m), float(m) + 0.25]:
             k = u / Nt
@@ -84,6 +90,14 @@ def main():
         for n in [0, 1, 7, Nt // 8, Nt // 5, Nt // 4 - 1, -(Nt // 4)]:
             out["window_hat"].append({"n": n, "Nt": Nt, "m": m, "sigma": sigma,
                                       "value": float(psi_hat(n, Nt, m, sigma))})
+    for (m, sigma) in [(4, 2.0), (6, 1.25), (2, 2.0), (8, 2.0)]:
+        b = float(mp.pi * (2 - 1 / mp.mpf(sigma)))
+        # includes points either side of b s = 1/2 (series / closed form switch)
+        s_half = 0.5 / b
+        u_sw = float(mp.sqrt(m ** 2 - mp.mpf(s_half) ** 2))
+        for u in [0.0, 0.25, -1.3, 2.5, m - 0.5, -(m - 0.01), u_sw - 1e-6, u_sw + 1e-6, m - 1e-4]:
+            out["window_du"].append({"u": u, "m": m, "sigma": sigma,
+                                     "value": float(psi_du(u, m, sigma))})
     rng = np.random.default_rng(20261202)
     for N, M in [((5,), 3), ((3, 4), 4), ((2, 3, 2), 3)]:
         D = len(N)
