@@ -14,7 +14,13 @@ NDFT on sampled outputs, per-stage device times come from HIP events recorded
 on the plan's stream inside a second timed pass, and the CPU baseline is the
 FP64 C++/OpenMP restatement of the reference plan (oracle/) on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Other workloads (--workload): "1d" -- N = 2^24, M = 2^24 nodes
+Beta(1/2, 1/2) - 1/2 clustered at the edges, m = 6, sigma = 1.25, TENSOR;
+"2d-spiral" -- N = 256^2, M = 65,536 nodes on an 8-interleave Archimedean
+spiral, m = 4, sigma = 2, POLYNOMIAL, where one step also computes the node
+gradient of ||A(k) f - y||^2.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3d|1d|2d-spiral]
 """
 import argparse
 import ctypes
@@ -31,7 +37,7 @@ import numpy as np  # noqa: E402
 HBM_PEAK = 8.0e12  # B/s, MI355X
 
 STAGES = ("deconvolve", "fft_forward", "interpolation", "grid_clear", "spreading", "fft_backward",
-          "deconvolve_transpose")
+          "deconvolve_transpose", "node_gradient")
 
 
 def stage_bytes(D, P, Q, M, sc, sk=8):
@@ -44,7 +50,41 @@ def stage_bytes(D, P, Q, M, sc, sk=8):
         "spreading": M * (D * sk + sc) + Q * sc,
         "fft_backward": 2 * Q * sc,
         "deconvolve_transpose": 2 * P * sc,
+        "node_gradient": M * (D * sk + sc + D * 8) + Q * sc,
     }
+
+
+def spiral_nodes(interleaves=8, samples=8192, turns=16):
+    t = np.arange(samples) / samples
+    ks = []
+    for q in range(interleaves):
+        th = 2 * np.pi * turns * t + 2 * np.pi * q / interleaves
+        ks.append(np.stack([0.5 * t * np.cos(th), 0.5 * t * np.sin(th)]))
+    return np.concatenate(ks, axis=1)
+
+
+def workload(name):
+    """(description, N, M, m, sigma, precompute name, rng, nodes)"""
+    if name == "3d":
+        rng = np.random.default_rng([123, 2])
+        N, M = (128, 128, 128), 2 ** 21
+        k = rng.random((3, M)) - 0.5
+        return ("3D NFFT N=128^3 M=2^21 uniform nodes, ComplexF32, Kaiser-Bessel m=4 sigma=2 "
+                "(grid 256^3), LINEAR table, forward+adjoint", N, M, 4, 2.0, "LINEAR", rng, k)
+    if name == "1d":
+        rng = np.random.default_rng([123, 5])
+        N, M = (2 ** 24,), 2 ** 24
+        k = (rng.beta(0.5, 0.5, size=M) - 0.5).reshape(1, M)
+        return ("1D NFFT N=2^24 M=2^24 nodes Beta(1/2,1/2)-1/2 (edge-clustered), ComplexF32, "
+                "Kaiser-Bessel m=6 sigma=1.25 (grid 20971520), TENSOR, forward+adjoint",
+                N, M, 6, 1.25, "TENSOR", rng, k)
+    if name == "2d-spiral":
+        rng = np.random.default_rng([123, 4])
+        k = spiral_nodes()
+        return ("2D NFFT N=256^2 M=65536 nodes on an 8-interleave Archimedean spiral, ComplexF32, "
+                "Kaiser-Bessel m=4 sigma=2, POLYNOMIAL, forward+adjoint+node gradient",
+                (256, 256), k.shape[1], 4, 2.0, "POLYNOMIAL", rng, k)
+    raise SystemExit(f"unknown workload {name}")
 
 
 def log(msg):
@@ -59,6 +99,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--ndft-fwd", type=int, default=1024, help="sampled nodes for the NDFT check")
     ap.add_argument("--ndft-adj", type=int, default=64, help="sampled grid points for the NDFT check")
+    ap.add_argument("--workload", default="3d", choices=("3d", "1d", "2d-spiral"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -73,25 +114,29 @@ def main():
     dev = torch.device("cuda", 0)
     L = nfftm._L.load()
 
-    N, M, m, sigma = (128, 128, 128), 2 ** 21, 4, 2.0
+    desc, N, M, m, sigma, pre_name, rng, k = workload(args.workload)
+    pre = getattr(nfftm, pre_name)
+    grad = args.workload == "2d-spiral"
     D = len(N)
     P = int(np.prod(N))
-    rng = np.random.default_rng([123, 2])
-    k = rng.random((D, M)) - 0.5
     f_h = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
     fh_h = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
 
     log("bench: creating FP32 and FP64 plans")
-    p32 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=nfftm.LINEAR, dtype=torch.float32)
+    p32 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float32)
     Q = int(np.prod(p32.Ntilde))
     f32 = torch.as_tensor(f_h, device=dev)
     fh32 = torch.as_tensor(fh_h, device=dev)
     y32 = torch.empty(M, dtype=torch.complex64, device=dev)
     x32 = torch.empty(N, dtype=torch.complex64, device=dev)
+    kb32 = torch.empty((M, D), dtype=torch.float64, device=dev)
 
     def step32():
         p32.forward(f32, y32)
         p32.adjoint(fh32, x32)
+        if grad:  # node cotangent of ||A(k) f - y||^2 with y := fHat (same buffers)
+            L.nfftm_grad_nodes(p32._handle, ctypes.c_void_p(f32.data_ptr()),
+                               ctypes.c_void_p(fh32.data_ptr()), ctypes.c_void_p(kb32.data_ptr()))
 
     for _ in range(args.warmup):
         step32()
@@ -111,18 +156,24 @@ def main():
     ms = (ctypes.c_double * 8)()
     cnt = (ctypes.c_int64 * 8)()
     L.nfftm_profile_read(p32._handle, ms, cnt)
+    # mean device time per launch of each stage (the FFT also runs inside the
+    # node gradient, so its count is per launch, not per step)
     stage_ms = {s: (ms[i] / cnt[i] if cnt[i] else None) for i, s in enumerate(STAGES)}
 
     # FP64 plan on the same workload
-    p64 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=nfftm.LINEAR, dtype=torch.float64)
+    p64 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float64)
     f64 = f32.to(torch.complex128)
     fh64 = fh32.to(torch.complex128)
     y64 = torch.empty(M, dtype=torch.complex128, device=dev)
     x64 = torch.empty(N, dtype=torch.complex128, device=dev)
+    kb64 = torch.empty((M, D), dtype=torch.float64, device=dev)
 
     def step64():
         p64.forward(f64, y64)
         p64.adjoint(fh64, x64)
+        if grad:
+            L.nfftm_grad_nodes(p64._handle, ctypes.c_void_p(f64.data_ptr()),
+                               ctypes.c_void_p(fh64.data_ptr()), ctypes.c_void_p(kb64.data_ptr()))
 
     for _ in range(max(1, args.warmup)):
         step64()
@@ -144,22 +195,29 @@ def main():
         "fwd_f32_vs_f64": float(np.linalg.norm(y32n - y64n) / np.linalg.norm(y64n)),
         "adj_f32_vs_f64": float(np.linalg.norm(x32n - x64n) / np.linalg.norm(x64n)),
     }
+    if grad:
+        a, b = kb32.cpu().numpy(), kb64.cpu().numpy()
+        err["grad_f32_vs_f64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     import oracle
 
-    sel = np.sort(rng.choice(M, size=args.ndft_fwd, replace=False))
+    # the direct sums cost P (resp. M) terms per sample: keep them to seconds
+    nfwd = min(args.ndft_fwd, max(16, int(2e9 // P)))
+    nadj = min(args.ndft_adj, max(16, int(2e9 // M)))
+    sel = np.sort(rng.choice(M, size=nfwd, replace=False))
     ref = oracle.ndft(k, f_h.astype(np.complex128), sel)
     err["fwd_f32_vs_ndft"] = float(np.linalg.norm(y32n[sel] - ref) / np.linalg.norm(ref))
     err["fwd_f64_vs_ndft"] = float(np.linalg.norm(y64n[sel] - ref) / np.linalg.norm(ref))
-    gsel = np.sort(rng.choice(P, size=args.ndft_adj, replace=False))
+    gsel = np.sort(rng.choice(P, size=nadj, replace=False))
     ref = oracle.ndft_adjoint(k, N, fh_h.astype(np.complex128), gsel)
     err["adj_f32_vs_ndft"] = float(np.linalg.norm(x32n.ravel()[gsel] - ref) / np.linalg.norm(ref))
     err["adj_f64_vs_ndft"] = float(np.linalg.norm(x64n.ravel()[gsel] - ref) / np.linalg.norm(ref))
-    err["samples"] = {"fwd_nodes": args.ndft_fwd, "adj_grid_points": args.ndft_adj}
+    err["samples"] = {"fwd_nodes": nfwd, "adj_grid_points": nadj}
 
     # roofline of the dominant hand-written kernel
     sc = 8
     sb = stage_bytes(D, P, Q, M, sc)
-    mine = ("deconvolve", "interpolation", "spreading", "deconvolve_transpose", "grid_clear")
+    mine = ("deconvolve", "interpolation", "spreading", "deconvolve_transpose", "grid_clear",
+            "node_gradient")
     dom = max((s for s in mine if stage_ms[s]), key=lambda s: stage_ms[s])
     achieved = sb[dom] / (stage_ms[dom] * 1e-3)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
@@ -171,7 +229,7 @@ def main():
     if not args.no_cpu:
         threads = oracle.lib().orc_num_threads()
         log(f"bench: CPU baseline (oracle restatement, {threads} threads)")
-        op = oracle.OraclePlan(k, N, m, sigma, oracle.LINEAR, True, workers=threads)
+        op = oracle.OraclePlan(k, N, m, sigma, getattr(oracle, pre_name), True, workers=threads)
         fc = f_h.astype(np.complex128)
         fhc = fh_h.astype(np.complex128)
         op.forward(fc)
@@ -186,7 +244,7 @@ def main():
             best = min(best, time.perf_counter() - t0)
             reps += 1
         cpu = {"value": M / best, "unit": "NU points/s", "cores": threads, "kind": "port",
-               "sample": f"full workload (N=128^3, M=2^21, FP64 blocked LINEAR plan), min of {reps} "
+               "sample": f"full workload ({desc}; FP64 blocked {pre_name} plan), min of {reps} "
                          "forward+adjoint runs, FFT by scipy.fft"}
 
     out = {
@@ -202,9 +260,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "3D NFFT N=128^3 M=2^21 uniform nodes, ComplexF32, Kaiser-Bessel m=4 "
-                               "sigma=2 (grid 256^3), LINEAR table, forward+adjoint",
-                   "N": list(N), "M": M, "m": m, "sigma": sigma, "precompute": "LINEAR"},
+        "config": {"workload": desc, "N": list(N), "M": M, "m": m, "sigma": sigma,
+                   "precompute": pre_name},
         "hbm_frac_total": B_total / t32 / HBM_PEAK,
         "fp64": {"value": M / t64, "ms_per_step": t64 * 1e3},
         "f32_over_f64": t64 / t32,

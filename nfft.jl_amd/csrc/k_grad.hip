@@ -54,7 +54,7 @@ __device__ __forceinline__ void kb_value_deriv(double u, double m, double b, dou
 template <int W>
 __device__ __forceinline__ int grad_weights(const KArgs& a, int d, double ksv, double* w, double* dw) {
   const double Ntd = a.Ntd[d];
-  const double kscale = ksv * Ntd;
+  const double kscale = scaled_node(ksv, Ntd);
   const long long c = (long long)kscale;
   const long long off = c - a.m + 1;
   const double t = kscale - (double)off;

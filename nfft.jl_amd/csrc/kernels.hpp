@@ -37,6 +37,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace nfftm {
 
 constexpr int MAXD = 8;
@@ -109,12 +111,21 @@ struct KArgs {
   int maxcand;         // covering (tile, offset) pairs per dim (fold)
 };
 
+// k * Ntilde rounded to FP64 before any further use, as the reference
+// computes kscale (src/precomputation.jl:538): kept out of a fused multiply-add
+// with the following subtraction, which at Ntilde ~ 2e7 would move the window
+// argument by ~1e-9 grid units.
+__device__ __forceinline__ double scaled_node(double ksv, double Ntd) {
+#pragma clang fp contract(off)
+  return ksv * Ntd;
+}
+
 // Window values of one node along dimension d.  Returns the wrapped cell
 // floor(k * Ntilde) (in [0, Ntilde)); t = kscale - off with
 // off = cell - m + 1 exactly as src/precomputation.jl:537-545.
 template <typename R, int W>
 __device__ __forceinline__ int node_weights(const KArgs& a, int d, double ksv, R* w) {
-  const double kscale = ksv * a.Ntd[d];
+  const double kscale = scaled_node(ksv, a.Ntd[d]);
   const long long c = (long long)kscale;  // unsafe_trunc, kscale >= 0
   const long long off = c - a.m + 1;
   const double t = kscale - (double)off;  // in [m-1, m), exact
@@ -246,7 +257,9 @@ __device__ __forceinline__ void gbl_add(R* p, R v) {
 // ---------------------------------------------------------------------------
 template <typename R, int D, int W>
 struct TileLayout {
-  static __host__ __device__ size_t wts_off(int tileElems) { return 2 * (size_t)tileElems * sizeof(R); }
+  // the spreading tile of D <= 2 accumulates in FP64 in both precisions
+  static constexpr size_t ACC = D <= 2 ? sizeof(double) : sizeof(R);
+  static __host__ __device__ size_t wts_off(int tileElems) { return 2 * (size_t)tileElems * ACC; }
   static __host__ __device__ size_t fv_off(int tileElems, int NB) {
     size_t o = wts_off(tileElems) + (size_t)NB * D * W * sizeof(R);
     return (o + 15) & ~(size_t)15;
@@ -264,7 +277,8 @@ struct TileLayout {
 
 // The same layout size for a runtime stencil width (host side).
 __host__ inline size_t tile_layout_bytes(int D, size_t sizeofR, int W, int tileElems, int NB) {
-  size_t o = 2 * (size_t)tileElems * sizeofR + (size_t)NB * D * W * sizeofR;
+  const size_t acc = D <= 2 ? sizeof(double) : sizeofR;
+  size_t o = 2 * (size_t)tileElems * acc + (size_t)NB * D * W * sizeofR;
   o = (o + 15) & ~(size_t)15;
   return o + (size_t)NB * 2 * sizeofR + (size_t)NB * D * sizeof(int) + (size_t)NB * sizeof(int);
 }
@@ -353,6 +367,9 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
                                                     typename cpx<R>::T* __restrict__ unused) {
   using C = typename cpx<R>::T;
   using L = TileLayout<R, D, W>;
+  // Accumulator: FP64 LDS adds for D <= 2 (FP32 products, one rounding per
+  // term, no range bound); packed fixed point for FP32 in 3-D.
+  using A = typename std::conditional<(D <= 2), double, R>::type;
   constexpr int NB = Batch<D, R>::NB;
   constexpr int NW = NT / 64;
   extern __shared__ __align__(16) char smem[];
@@ -370,7 +387,7 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
 
   // FP32: fixed-point scale s = 2^(30 - ceil(log2 B)), B = sum_j |f_j|_inf * wmaxD
   R fscale = 1, inv = 1;
-  if (sizeof(R) == 4) {
+  if (sizeof(A) == 4) {
     R part = 0;
     for (int j = it.y + threadIdx.x; j < it.z; j += NT) {
       const C v = fhat[a.perm[j]];
@@ -389,7 +406,7 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
     inv = (R)ldexp(1.0, e - 30);
   }
 
-  for (int i = threadIdx.x; i < 2 * a.tileElems; i += NT) ((R*)tile)[i] = 0;
+  for (int i = threadIdx.x; i < 2 * a.tileElems; i += NT) ((A*)tile)[i] = 0;
 
   for (int b0 = it.y; b0 < it.z; b0 += NB) {
     const int nb = min(NB, it.z - b0);
@@ -411,7 +428,7 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
           const int base = (bx + lx) + a.Sx * (by + ly) + a.Sxy * bz;
 #pragma unroll
           for (int lz = 0; lz < W; ++lz)
-            SpreadCell<R>::add(tile, a.tileElems, base + a.Sxy * lz, vr * w3[lz], vi * w3[lz]);
+            SpreadCell<A>::add(tile, a.tileElems, base + a.Sxy * lz, vr * w3[lz], vi * w3[lz]);
         }
       }
     } else {
@@ -430,10 +447,10 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
 #pragma unroll
             for (int ly = 0; ly < W; ++ly) {
               const R w2 = w[W + ly];
-              SpreadCell<R>::add(tile, a.tileElems, base + a.Sx * ly, vr * w2, vi * w2);
+              SpreadCell<A>::add(tile, a.tileElems, base + a.Sx * ly, (A)(vr * w2), (A)(vi * w2));
             }
           } else {
-            SpreadCell<R>::add(tile, a.tileElems, loc[n] + lx, vr, vi);
+            SpreadCell<A>::add(tile, a.tileElems, loc[n] + lx, (A)vr, (A)vi);
           }
         }
       }
@@ -455,7 +472,8 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
         pz = q2;
       }
     }
-    out[i] = SpreadCell<R>::get(tile, a.tileElems, px + a.Sx * py + a.Sxy * pz, inv);
+    const auto v = SpreadCell<A>::get(tile, a.tileElems, px + a.Sx * py + a.Sxy * pz, inv);
+    out[i] = C{(R)v.x, (R)v.y};
   }
 }
 

@@ -143,9 +143,11 @@ def test_autograd_function_and_rrule():
     assert k.grad.device.type == "cpu" and k.grad.dtype == torch.float64
     assert rel(k.grad, kbar) < 1e-14
     assert rel(x.grad, xbar) < 1e-14
+    # against the exact (direct-sum) operator: the m = 4 window's accuracy,
+    # ~1e-7 for the transform and ~1e-6 for its node derivative
     xn = x.detach().cpu().numpy()
     assert rel(x.grad, O.ndft_adjoint(kn, N, 2 * r.cpu().numpy())) < 1e-6
-    assert rel(k.grad, O.ndft_grad(kn, xn, 2 * r.cpu().numpy())) < 1e-6
+    assert rel(k.grad, O.ndft_grad(kn, xn, 2 * r.cpu().numpy())) < 1e-5
 
 
 def test_grad_edge_cases():

@@ -1,7 +1,7 @@
 """GPU parity: libnfftm.so (through the nfftm host package and its C ABI)
 against the CPU oracle (FP64 restatement of NFFT.jl) and the direct NDFT.
 
-Tolerances (written here, per BASELINE): FP64 GPU vs oracle rel-l2 <= 1e-12;
+Tolerances (written here): FP64 GPU vs oracle rel-l2 <= 1e-12;
 FP32 GPU vs FP64 NDFT rel-l2 <= 1e-5; the reference's own accuracy contract
 (test/accuracy.jl:46-47, test/gpu.jl:22): Kaiser-Bessel, m = 5, sigma = 2,
 rel-l2 vs NDFT < 1e-7 in FP64.
