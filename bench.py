@@ -20,7 +20,13 @@ Beta(1/2, 1/2) - 1/2 clustered at the edges, m = 6, sigma = 1.25, TENSOR;
 spiral, m = 4, sigma = 2, POLYNOMIAL, where one step also computes the node
 gradient of ||A(k) f - y||^2.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3d|1d|2d-spiral]
+"batch" -- 64 2-D plans with N drawn from {64^2, 128^2, 256^2, 512^2} and M
+log-uniform in [1e4, 1e6] (uniform nodes), plus 32 images sharing one 256^2
+golden-angle radial trajectory (402 spokes x 512 samples), m = 4, sigma = 2,
+LINEAR, run as one autotuned batch (persistent launches, HIP graphs) and, for
+comparison, plan by plan.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3d|1d|2d-spiral|batch]
 """
 import argparse
 import ctypes
@@ -91,6 +97,121 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def radial_nodes(spokes, samples):
+    phi = (1 + 5 ** 0.5) / 2
+    th = np.arange(spokes) * np.pi / phi
+    r = -0.5 + np.arange(samples) / samples
+    return np.stack([np.outer(np.cos(th), r).ravel(), np.outer(np.sin(th), r).ravel()])
+
+
+def run_batch(args, torch, nfftm, dev):
+    """Batched uneven plans plus shared-trajectory images (see the module
+    docstring).  value = total NU points of all slots per second of batch
+    forward + adjoint."""
+    rng = np.random.default_rng([123, 3])
+    specs = []
+    for _ in range(64):
+        n = int(rng.choice([64, 128, 256, 512]))
+        M = int(round(10 ** rng.uniform(4, 6)))
+        specs.append(((n, n), rng.random((2, M)) - 0.5, 1))
+    specs.append(((256, 256), radial_nodes(402, 512), 32))
+
+    def build(dtype):
+        plans = [nfftm.plan_nfft(k, N, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=dtype)
+                 for N, k, _ in specs]
+        return plans
+
+    def data(plans, b, cd, seed):
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        # drawn in FP64 and rounded, so both precisions see the same data
+        fs = [torch.randn(plans[i].N, dtype=torch.complex128, device=dev, generator=g).to(cd)
+              for i, _ in b.slots]
+        fh = [torch.randn((plans[i].J,), dtype=torch.complex128, device=dev, generator=g).to(cd)
+              for i, _ in b.slots]
+        return fs, fh
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    res = {}
+    for name, dtype, cd in (("f32", torch.float32, torch.complex64), ("f64", torch.float64, torch.complex128)):
+        log(f"bench: batch {name}: plans")
+        plans = build(dtype)
+        t0 = time.perf_counter()
+        b = nfftm.BatchPlan(plans, [n for _, _, n in specs], autotune=True)
+        t_setup = time.perf_counter() - t0
+        fs, fh = data(plans, b, cd, 5)
+        ys = [torch.empty((plans[i].J,), dtype=cd, device=dev) for i, _ in b.slots]
+        xs = [torch.empty(plans[i].N, dtype=cd, device=dev) for i, _ in b.slots]
+
+        def step_batch():
+            b.forward(fs, ys)
+            b.adjoint(fh, xs)
+
+        def step_single():
+            for s, (i, _) in enumerate(b.slots):
+                plans[i].forward(fs[s], ys[s])
+                plans[i].adjoint(fh[s], xs[s])
+
+        log(f"bench: batch {name}: timing")
+        tb = timed(step_batch, args.steps, args.warmup)
+        yb = [y.clone() for y in ys]
+        xb = [x.clone() for x in xs]
+        ts = timed(step_single, max(2, args.steps // 4), 1)
+        err = max(max(float(torch.linalg.vector_norm(a - c) / torch.linalg.vector_norm(c))
+                      for a, c in zip(yb, ys)),
+                  max(float(torch.linalg.vector_norm(a - c) / torch.linalg.vector_norm(c))
+                      for a, c in zip(xb, xs)))
+        shapes = sorted({(plans[i].Ntilde, plans[i].tile_shape()) for i, _ in b.slots})
+        res[name] = {"t_batch": tb, "t_single": ts, "setup_s": t_setup, "graph": b.graph,
+                     "batch_vs_single_relerr": err, "tiles": [str(x) for x in shapes],
+                     "y": [y.cpu() for y in (yb[0], yb[-1])], "x": [x.cpu() for x in (xb[0], xb[-1])]}
+        del b, plans, fs, fh, ys, xs
+        torch.cuda.empty_cache()
+    Mtot = sum(k.shape[1] * n for _, k, n in specs)
+    e32 = max(float(np.linalg.norm(a.numpy() - c.numpy()) / np.linalg.norm(c.numpy()))
+              for a, c in zip(res["f32"]["y"] + res["f32"]["x"], res["f64"]["y"] + res["f64"]["x"]))
+    out = {
+        "metric": "NU points/s fwd+adj, batch of 64 uneven 2-D plans + 32 radial images (FP32)",
+        "value": Mtot / res["f32"]["t_batch"],
+        "unit": "NU points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["f32"]["t_batch"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "64 2-D plans N in {64,128,256,512}^2, M log-uniform [1e4,1e6], uniform "
+                               "nodes; + 32 images on a 256^2 radial trajectory (402x512, golden angle); "
+                               "m=4 sigma=2 LINEAR, autotuned, one batch, forward+adjoint",
+                   "slots": 96, "total_nodes": Mtot},
+        "single_plans_ms_per_step": res["f32"]["t_single"] * 1e3,
+        "batch_speedup_vs_single": res["f32"]["t_single"] / res["f32"]["t_batch"],
+        "fp64": {"value": Mtot / res["f64"]["t_batch"], "ms_per_step": res["f64"]["t_batch"] * 1e3},
+        "f32_over_f64": res["f64"]["t_batch"] / res["f32"]["t_batch"],
+        "errors": {"f32_vs_f64_sampled_slots": e32,
+                   "batch_vs_single_f32": res["f32"]["batch_vs_single_relerr"],
+                   "batch_vs_single_f64": res["f64"]["batch_vs_single_relerr"]},
+        "graph": res["f32"]["graph"],
+        "setup_s": {"f32": res["f32"]["setup_s"], "f64": res["f64"]["setup_s"]},
+        "tiles_f32": res["f32"]["tiles"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +220,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--ndft-fwd", type=int, default=1024, help="sampled nodes for the NDFT check")
     ap.add_argument("--ndft-adj", type=int, default=64, help="sampled grid points for the NDFT check")
-    ap.add_argument("--workload", default="3d", choices=("3d", "1d", "2d-spiral"))
+    ap.add_argument("--workload", default="3d", choices=("3d", "1d", "2d-spiral", "batch"))
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 256 MiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -113,6 +236,8 @@ def main():
     torch.cuda.init()
     dev = torch.device("cuda", 0)
     L = nfftm._L.load()
+    if args.workload == "batch":
+        return run_batch(args, torch, nfftm, dev)
 
     desc, N, M, m, sigma, pre_name, rng, k = workload(args.workload)
     pre = getattr(nfftm, pre_name)
@@ -271,6 +396,14 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    if args.copy_calib:
+        # a device-to-device copy of exactly 256 MiB read + 256 MiB written:
+        # the counters it produces calibrate FETCH_SIZE / WRITE_SIZE per byte
+        src = torch.ones(2 ** 26, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)  # one elementwise kernel: reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

@@ -140,6 +140,41 @@ int nfftm_set_timing(nfftm_plan p, int on);
 int nfftm_profile(nfftm_plan p, int on);
 int nfftm_profile_read(nfftm_plan p, double ms[8], int64_t count[8]);
 
+/* Autotuner: times the plan's gridding kernels (spreading with its fold,
+ * plus interpolation) on its own nodes for each candidate tile edge (D = 1:
+ * 512..8192, D = 2: 16..128) and workgroup size (256, 512; D = 3: 512, 1024)
+ * and keeps the fastest.  Synchronises; results are unchanged (same
+ * arithmetic, different tiling).  No counterpart in the reference, whose
+ * block size is fixed per dimension (src/precomputation.jl:59-77). */
+int nfftm_plan_tune(nfftm_plan p);
+
+/* Current tile edge per dimension (0 beyond D, or when the plan uses the
+ * node-parallel kernels) and workgroup size of the gridding kernels. */
+int nfftm_tile_shape(nfftm_plan p, int tile[3], int* threads);
+
+/* Batches: many transforms per launch.  A batch takes plans of one precision
+ * and D <= 2 (different N, M and nodes) and, per plan, ntrans[i] >= 1 images
+ * that share that plan's nodes (NULL: one each).  Slot s = (plan i, image c)
+ * in order of i, then c.  Every stage of all slots runs as one persistent
+ * launch striding over the union of their work items (tiles of all slots in
+ * one spreading / interpolation launch, one deconvolution launch, one fold),
+ * the FFTs as one batched rocFFT per distinct grid size, and each direction
+ * is captured once in a HIP graph and replayed.  Data pointers are passed per
+ * call (f_dev[s], fhat_dev[s]: device buffers of slot s).  The plans must
+ * outlive the batch; nfftm_nodes on a member plan invalidates it (its calls
+ * then fail with NFFTM_ERR_ARG).  The reference transforms one image per
+ * mul! (src/implementation.jl:155-193); a batch computes exactly the same
+ * values as ntrans[i] such calls per plan.
+ * flags: NFFTM_BATCH_AUTOTUNE (run nfftm_plan_tune on every plan first),
+ * NFFTM_BATCH_NO_GRAPH (launch directly instead of replaying graphs). */
+typedef struct nfftm_batch_s* nfftm_batch;
+enum { NFFTM_BATCH_AUTOTUNE = 1, NFFTM_BATCH_NO_GRAPH = 2 };
+int nfftm_batch_create(nfftm_batch* out, int nplans, const nfftm_plan* plans, const int* ntrans, int flags);
+int nfftm_batch_info(nfftm_batch b, int* nslots, int* graph);
+int nfftm_batch_forward(nfftm_batch b, const void* const* f_dev, void* const* fhat_dev, void* hip_stream);
+int nfftm_batch_adjoint(nfftm_batch b, const void* const* fhat_dev, void* const* f_dev, void* hip_stream);
+void nfftm_batch_destroy(nfftm_batch b);
+
 /* Change the HIP stream the plan enqueues on. */
 int nfftm_set_stream(nfftm_plan p, void* hip_stream);
 

@@ -18,14 +18,14 @@ __device__ __forceinline__ int img_index(int g, int N, int Nt, int Na) {
   return -1;
 }
 
+// Grid points 2i and 2i + 1 (i < Q / 2).
 template <typename R>
-__global__ __launch_bounds__(256) void k_deconv(DArgs a, const typename cpx<R>::T* __restrict__ f,
-                                                typename cpx<R>::T* __restrict__ g) {
+__device__ __forceinline__ void deconv_pair(const DArgs& a, long long i, const typename cpx<R>::T* __restrict__ f,
+                                            typename cpx<R>::T* __restrict__ g) {
   using C = typename cpx<R>::T;
   const int Nt0 = a.Nt[0], N0 = a.N[0], Na0 = a.Na[0];
   const unsigned half = (unsigned)(Nt0 >> 1);
-  const long long npairs = a.Q >> 1;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < npairs; i += (long long)gridDim.x * 256) {
+  {
     const unsigned row = fdiv((unsigned)(i >> 0), a.div_half);  // i < 2^31 (checked on the host)
     const int x = 2 * (int)(i - (long long)row * half);
     bool inside = true;
@@ -67,34 +67,77 @@ __global__ __launch_bounds__(256) void k_deconv(DArgs a, const typename cpx<R>::
   }
 }
 
-// deconvolve_transpose! (src/deconvolution.jl:69-92): f_n = g[(n mod Nt)] *
-// prod_d 1/psi_hat_d(n_d).
 template <typename R>
-__global__ __launch_bounds__(256) void k_deconv_t(DArgs a, const typename cpx<R>::T* __restrict__ g,
-                                                  typename cpx<R>::T* __restrict__ f) {
+__global__ __launch_bounds__(256) void k_deconv(DArgs a, const typename cpx<R>::T* __restrict__ f,
+                                                typename cpx<R>::T* __restrict__ g) {
+  const long long npairs = a.Q >> 1;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < npairs; i += (long long)gridDim.x * 256)
+    deconv_pair<R>(a, i, f, g);
+}
+
+// deconvolve_transpose! (src/deconvolution.jl:69-92): f_n = g[(n mod Nt)] *
+// prod_d 1/psi_hat_d(n_d).  One image row: the outer-dimension offset and
+// factor product, then the threads of the workgroup along dimension 1.
+template <typename R>
+__device__ __forceinline__ void deconv_t_row(const DArgs& a, int row, const typename cpx<R>::T* __restrict__ g,
+                                             typename cpx<R>::T* __restrict__ f, int i0, int step) {
   using C = typename cpx<R>::T;
-  const int i0 = blockIdx.x * 256 + threadIdx.x;
   const int N0 = a.N[0], Na0 = a.Na[0];
-  const int nrows = (int)(a.P / N0);
-  for (int row = blockIdx.y; row < nrows; row += gridDim.y) {
-    long long go = 0;
-    double facr = 1.0;
-    int r = row;
-    for (int d = 1; d < a.D; ++d) {
-      const int id = r % a.N[d];
-      r /= a.N[d];
-      const int gd = id < a.Na[d] ? id - a.Na[d] + a.Nt[d] : id - a.Na[d];
-      go += (long long)gd * a.gstr[d];
-      facr *= a.fac[d][id];
-    }
-    if (i0 >= N0) continue;
-    const int gx = i0 < Na0 ? i0 - Na0 + a.Nt[0] : i0 - Na0;
+  long long go = 0;
+  double facr = 1.0;
+  int r = row;
+  for (int d = 1; d < a.D; ++d) {
+    const int id = r % a.N[d];
+    r /= a.N[d];
+    const int gd = id < a.Na[d] ? id - a.Na[d] + a.Nt[d] : id - a.Na[d];
+    go += (long long)gd * a.gstr[d];
+    facr *= a.fac[d][id];
+  }
+  for (int i = i0; i < N0; i += step) {
+    const int gx = i < Na0 ? i - Na0 + a.Nt[0] : i - Na0;
     const C v = g[go + gx];
-    const double s = a.fac[0][i0] * facr * a.oscale;
+    const double s = a.fac[0][i] * facr * a.oscale;
     C out;
     out.x = (R)((double)v.x * s);
     out.y = (R)((double)v.y * s);
-    f[(long long)row * N0 + i0] = out;
+    f[(long long)row * N0 + i] = out;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_deconv_t(DArgs a, const typename cpx<R>::T* __restrict__ g,
+                                                  typename cpx<R>::T* __restrict__ f) {
+  const int i0 = blockIdx.x * 256 + threadIdx.x;
+  const int nrows = (int)(a.P / a.N[0]);
+  for (int row = blockIdx.y; row < nrows; row += gridDim.y)
+    if (i0 < a.N[0]) deconv_t_row<R>(a, row, g, f, i0, a.N[0]);
+}
+
+// Batch forms: work entries {slot, begin, end, 0} over grid-point pairs
+// (deconvolve) or image rows (transpose), strided over a resident grid.
+template <typename R>
+__global__ __launch_bounds__(256) void k_deconv_batch(const DArgs* __restrict__ da, const int4* __restrict__ work,
+                                                      int nwork, const void* const* __restrict__ in,
+                                                      void* const* __restrict__ out) {
+  using C = typename cpx<R>::T;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int4 e = work[w];
+    const DArgs& a = da[e.x];
+    for (long long i = (long long)e.y + threadIdx.x; i < e.z; i += 256)
+      deconv_pair<R>(a, i, (const C*)in[e.x], (C*)out[e.x]);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_deconv_t_batch(const DArgs* __restrict__ da, const int4* __restrict__ work,
+                                                        int nwork, const void* const* __restrict__ in,
+                                                        void* const* __restrict__ out) {
+  using C = typename cpx<R>::T;
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int4 e = work[w];
+    const DArgs& a = da[e.x];
+    for (int row = e.y; row < e.z; ++row)
+      deconv_t_row<R>(a, row, (const C*)in[e.x], (C*)out[e.x], threadIdx.x, 256);
   }
 }
 
@@ -225,6 +268,22 @@ hipError_t launch_deconv_t(bool dbl, const DArgs& a, hipStream_t s, const void* 
     hipLaunchKernelGGL(k_deconv_t<double>, grid, blk, 0, s, a, (const double2*)g, (double2*)f);
   else
     hipLaunchKernelGGL(k_deconv_t<float>, grid, blk, 0, s, a, (const float2*)g, (float2*)f);
+  return hipGetLastError();
+}
+
+hipError_t launch_deconv_batch(bool dbl, int nblk, hipStream_t s, const DArgs* da, const int4* work, int nwork,
+                               const void* const* in, void* const* out) {
+  if (nwork == 0) return hipSuccess;
+  if (dbl) hipLaunchKernelGGL(k_deconv_batch<double>, dim3(nblk), dim3(256), 0, s, da, work, nwork, in, out);
+  else hipLaunchKernelGGL(k_deconv_batch<float>, dim3(nblk), dim3(256), 0, s, da, work, nwork, in, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_deconv_t_batch(bool dbl, int nblk, hipStream_t s, const DArgs* da, const int4* work, int nwork,
+                                 const void* const* in, void* const* out) {
+  if (nwork == 0) return hipSuccess;
+  if (dbl) hipLaunchKernelGGL(k_deconv_t_batch<double>, dim3(nblk), dim3(256), 0, s, da, work, nwork, in, out);
+  else hipLaunchKernelGGL(k_deconv_t_batch<float>, dim3(nblk), dim3(256), 0, s, da, work, nwork, in, out);
   return hipGetLastError();
 }
 

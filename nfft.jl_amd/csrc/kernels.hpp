@@ -362,9 +362,12 @@ struct SpreadCell<double> {  // two FP64 planes
   }
 };
 
+// One work item (<= CH nodes of one tile) of spreading: the padded tile is
+// accumulated in LDS and written to the item's slot of `scratch`.
 template <typename R, int D, int W, int NT>
-__global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<R>::T* __restrict__ fhat,
-                                                    typename cpx<R>::T* __restrict__ unused) {
+__device__ __forceinline__ void spread_tile_item(const KArgs& a, int item,
+                                                 const typename cpx<R>::T* __restrict__ fhat,
+                                                 typename cpx<R>::T* __restrict__ scratch) {
   using C = typename cpx<R>::T;
   using L = TileLayout<R, D, W>;
   // Accumulator: FP64 LDS adds for D <= 2 (FP32 products, one rounding per
@@ -373,8 +376,6 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
   constexpr int NB = Batch<D, R>::NB;
   constexpr int NW = NT / 64;
   extern __shared__ __align__(16) char smem[];
-  const int item = xcd_item(a.nitems);
-  if (item >= a.nitems) return;
   const int4 it = a.items[item];
   char* tile = smem;
   R* wts = (R*)(smem + L::wts_off(a.tileElems));
@@ -459,7 +460,7 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
   __syncthreads();
 
   // padded tile -> this item's scratch slot (dense, px fastest), plain stores
-  C* out = (C*)a.scratch + (size_t)item * a.padCells;
+  C* out = scratch + (size_t)item * a.padCells;
   for (int i = threadIdx.x; i < a.padCells; i += NT) {
     int px = i, py = 0, pz = 0;
     if (D >= 2) {
@@ -475,6 +476,14 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
     const auto v = SpreadCell<A>::get(tile, a.tileElems, px + a.Sx * py + a.Sxy * pz, inv);
     out[i] = C{(R)v.x, (R)v.y};
   }
+}
+
+template <typename R, int D, int W, int NT>
+__global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<R>::T* __restrict__ fhat,
+                                                    typename cpx<R>::T* __restrict__ unused) {
+  const int item = xcd_item(a.nitems);
+  if (item >= a.nitems) return;
+  spread_tile_item<R, D, W, NT>(a, item, fhat, (typename cpx<R>::T*)a.scratch);
 }
 
 // Fold: grid point x sums, over the padded tiles covering it, the scratch
@@ -506,6 +515,38 @@ __device__ __forceinline__ int cover_pairs(const KArgs& a, int d, int x, int* tl
   return n;
 }
 
+// Sum at grid point x (dimension 1) of the scratch slots covering it, given
+// the covering (tile, offset) pairs of dimensions 2 and 3.
+template <typename R, int D>
+__device__ __forceinline__ typename cpx<R>::T fold_sum(const KArgs& a, const typename cpx<R>::T* __restrict__ scr,
+                                                       int x, int n1, const int* t1, const int* p1, int n2,
+                                                       const int* t2, const int* p2) {
+  using C = typename cpx<R>::T;
+  int t0[3] = {0, 0, 0}, p0[3] = {0, 0, 0};
+  const int n0 = cover_pairs(a, 0, x, t0, p0);
+  R sr = 0, si = 0;
+#pragma unroll
+  for (int c2 = 0; c2 < 3; ++c2) {
+    if (c2 >= n2) break;
+#pragma unroll
+    for (int c1 = 0; c1 < 3; ++c1) {
+      if (c1 >= n1) break;
+#pragma unroll
+      for (int c0 = 0; c0 < 3; ++c0) {
+        if (c0 >= n0) break;
+        const int t = t0[c0] + a.ntiles[0] * (t1[c1] + a.ntiles[1] * t2[c2]);
+        const int off = p0[c0] + a.Pd[0] * (p1[c1] + a.Pd[1] * p2[c2]);
+        for (int itm = a.tstart[t]; itm < a.tstart[t + 1]; ++itm) {
+          const C v = scr[(size_t)itm * a.padCells + off];
+          sr += v.x;
+          si += v.y;
+        }
+      }
+    }
+  }
+  return C{sr, si};
+}
+
 template <typename R, int D>
 __global__ __launch_bounds__(256) void k_fold(KArgs a, typename cpx<R>::T* __restrict__ grid) {
   using C = typename cpx<R>::T;
@@ -520,45 +561,23 @@ __global__ __launch_bounds__(256) void k_fold(KArgs a, typename cpx<R>::T* __res
     if (D >= 2) n1 = cover_pairs(a, 1, y, t1, p1);
     if (D == 3) n2 = cover_pairs(a, 2, z, t2, p2);
     if (x >= a.Nt[0]) continue;
-    int t0[3] = {0, 0, 0}, p0[3] = {0, 0, 0};
-    const int n0 = cover_pairs(a, 0, x, t0, p0);
-    R sr = 0, si = 0;
-#pragma unroll
-    for (int c2 = 0; c2 < 3; ++c2) {
-      if (c2 >= n2) break;
-#pragma unroll
-      for (int c1 = 0; c1 < 3; ++c1) {
-        if (c1 >= n1) break;
-#pragma unroll
-        for (int c0 = 0; c0 < 3; ++c0) {
-          if (c0 >= n0) break;
-          const int t = t0[c0] + a.ntiles[0] * (t1[c1] + a.ntiles[1] * t2[c2]);
-          const int off = p0[c0] + a.Pd[0] * (p1[c1] + a.Pd[1] * p2[c2]);
-          for (int itm = a.tstart[t]; itm < a.tstart[t + 1]; ++itm) {
-            const C v = scr[(size_t)itm * a.padCells + off];
-            sr += v.x;
-            si += v.y;
-          }
-        }
-      }
-    }
-    grid[(long long)row * a.Nt[0] + x] = C{sr, si};
+    grid[(long long)row * a.Nt[0] + x] = fold_sum<R, D>(a, scr, x, n1, t1, p1, n2, t2, p2);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Interpolation
 // ---------------------------------------------------------------------------
+// One work item of interpolation: stage the padded tile, gather every node.
 template <typename R, int D, int W, int NT>
-__global__ __launch_bounds__(NT) void k_interp_tile(KArgs a, const typename cpx<R>::T* __restrict__ grid,
-                                                    typename cpx<R>::T* __restrict__ fhat) {
+__device__ __forceinline__ void interp_tile_item(const KArgs& a, int item,
+                                                 const typename cpx<R>::T* __restrict__ grid,
+                                                 typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
   using L = TileLayout<R, D, W>;
   constexpr int NB = Batch<D, R>::NB;
   constexpr int NW = NT / 64;
   extern __shared__ __align__(16) char smem[];
-  const int item = xcd_item(a.nitems);
-  if (item >= a.nitems) return;
   const int4 it = a.items[item];
   C* tile = (C*)smem;
   R* wts = (R*)(smem + L::wts_off(a.tileElems));
@@ -649,6 +668,14 @@ __global__ __launch_bounds__(NT) void k_interp_tile(KArgs a, const typename cpx<
       }
     }
   }
+}
+
+template <typename R, int D, int W, int NT>
+__global__ __launch_bounds__(NT) void k_interp_tile(KArgs a, const typename cpx<R>::T* __restrict__ grid,
+                                                    typename cpx<R>::T* __restrict__ fhat) {
+  const int item = xcd_item(a.nitems);
+  if (item >= a.nitems) return;
+  interp_tile_item<R, D, W, NT>(a, item, grid, fhat);
 }
 
 }  // namespace nfftm

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace nfftm {
 
 // Arguments of the deconvolution kernels (src/deconvolution.jl).
@@ -39,6 +41,21 @@ hipError_t launch_generic(bool spread, bool dbl, int W, const KArgs& a, hipStrea
 // the transformed grid g of the forward path.
 hipError_t launch_grad_nodes(bool dbl, int W, const KArgs& a, hipStream_t s, const void* g,
                              const void* ybar, double* kbar);
+
+// Batch stages (k_batch.hip): persistent kernels striding over (slot, item)
+// work lists; ka holds one KArgs per slot in device memory.
+hipError_t launch_tile_batch(int D, bool spread, bool dbl, int W, int NT, int nblk, size_t lds, hipStream_t s,
+                             const KArgs* ka, const int2* work, int nwork, const void* const* in,
+                             void* const* out);
+hipError_t launch_fold_batch(int D, bool dbl, int nblk, hipStream_t s, const KArgs* ka, const int4* work,
+                             int nwork, void* const* grids);
+int tile_batch_blocks(int D, bool spread, bool dbl, int W, int NT, size_t lds);
+// work entries {slot, begin, end, 0}: element pairs of the grid (forward) or
+// image rows (transpose)
+hipError_t launch_deconv_batch(bool dbl, int nblk, hipStream_t s, const DArgs* da, const int4* work, int nwork,
+                               const void* const* in, void* const* out);
+hipError_t launch_deconv_t_batch(bool dbl, int nblk, hipStream_t s, const DArgs* da, const int4* work, int nwork,
+                                 const void* const* in, void* const* out);
 
 hipError_t launch_deconv(bool dbl, const DArgs& a, hipStream_t s, const void* f, void* g);
 hipError_t launch_deconv_t(bool dbl, const DArgs& a, hipStream_t s, const void* g, void* f);

@@ -223,7 +223,8 @@ struct FFTStage {
     return NFFTM_OK;
   }
 
-  int build(int D, const i64* Nt, bool dbl) {
+  // nbatch consecutive grids of prod(Nt) points (D <= 3 when nbatch > 1)
+  int build(int D, const i64* Nt, bool dbl, size_t nbatch = 1) {
     std::call_once(g_rocfft_once, [] { rocfft_setup(); });
     for (int dir = 0; dir < 2; ++dir) {
       auto& v = dir == 0 ? fwd : bwd;
@@ -238,7 +239,7 @@ struct FFTStage {
       }
       size_t outer = 1;
       for (int d = 3; d < D; ++d) outer *= (size_t)Nt[d];
-      int rc = make_pass(v, f, dbl, d3, len, str, s, outer, 1, 0);
+      int rc = make_pass(v, f, dbl, d3, len, str, s, outer * nbatch, 1, 0);
       if (rc) return rc;
       size_t inner = s;
       for (int d = 3; d < D; ++d) {
@@ -315,6 +316,7 @@ struct nfftm_plan_s {
   DevBuf<int> d_tstart;
   DevBuf<char> d_scratch;
   int nitems = 0;
+  int gen = 0;        // incremented whenever the node arrays are rebuilt
   double wmax = 1.0;  // bound of one (normalised) window value
 
   // tiling
@@ -470,45 +472,48 @@ int max_cover(i64 T, i64 P, i64 Nt, int m) {
   return best;
 }
 
-// Tile geometry of the gridding kernels (D <= 3).  Sx is the padded row
-// extent rounded to 8 (mod 16) when that costs at most one element, which
-// keeps the 8x8 lane footprint of a 3-D stencil on distinct LDS banks.
-void choose_tiles(nfftm_plan_s& p) {
+// Tile geometry of D <= 2 for tile edge t (clipped to the grid): padded
+// extents, LDS strides and bytes; the row stride Sx is the padded extent
+// rounded to 8 (mod 16) when that costs one element.  False if it exceeds the
+// LDS budget.
+bool set_tile_shape(nfftm_plan_s& p, int t) {
   const int D = p.D, W = p.W;
-  const bool dbl = p.dbl();
   const size_t budget = 158 * 1024;
-  auto fit = [&](int t0, int t1, int t2) {
-    int T[3] = {t0, t1, t2}, Pd[3] = {1, 1, 1};
-    for (int d = 0; d < D; ++d) {
-      T[d] = (int)std::min<i64>(T[d], p.Nt[d]);
-      Pd[d] = T[d] + W - 1;
-    }
-    int Sx = Pd[0];
-    if (D >= 2) {
-      const int r = Pd[0] % 16;
-      if (r == 7) Sx = Pd[0] + 1;
-    }
-    const int Sxy = Sx * (D >= 2 ? Pd[1] : 1);
-    const int te = Sxy * (D == 3 ? Pd[2] : 1);
-    const size_t bytes = tile_lds_bytes(D, dbl, W, te);
-    for (int d = 0; d < 3; ++d) {
-      p.T[d] = D > d ? T[d] : 1;
-      p.Pd[d] = D > d ? Pd[d] : 1;
-      p.ntiles[d] = D > d ? (int)((p.Nt[d] + T[d] - 1) / T[d]) : 1;
-    }
-    p.Sx = Sx;
-    p.Sxy = Sxy;
-    p.tileElems = te;
-    p.lds = bytes;
-    return bytes <= budget;
-  };
+  int T[3] = {t, D >= 2 ? t : 1, 1}, Pd[3] = {1, 1, 1};
+  for (int d = 0; d < D; ++d) {
+    T[d] = (int)std::min<i64>(T[d], p.Nt[d]);
+    Pd[d] = T[d] + W - 1;
+  }
+  int Sx = Pd[0];
+  if (D >= 2 && Pd[0] % 16 == 7) Sx = Pd[0] + 1;
+  const int Sxy = Sx * (D >= 2 ? Pd[1] : 1);
+  const int te = Sxy;
+  const size_t bytes = tile_lds_bytes(D, p.dbl(), W, te);
+  for (int d = 0; d < 3; ++d) {
+    p.T[d] = D > d ? T[d] : 1;
+    p.Pd[d] = D > d ? Pd[d] : 1;
+    p.ntiles[d] = D > d ? (int)((p.Nt[d] + T[d] - 1) / T[d]) : 1;
+  }
+  p.Sx = Sx;
+  p.Sxy = Sxy;
+  p.tileElems = te;
+  p.lds = bytes;
+  return bytes <= budget;
+}
+
+// Default tile shape: D = 3 uses the compile-time geometry of k_tile3.hip;
+// D = 2 tiles of 64^2 (halved while the grid has fewer than 256 tiles, and
+// until the tile fits LDS); D = 1 tiles of 4096 (2048 in FP64) likewise.
+// Grids smaller than a padded tile fail tiles_ok and take the node-parallel
+// path.
+void choose_tiles(nfftm_plan_s& p) {
+  const int D = p.D;
+  const bool dbl = p.dbl();
   if (D == 3) {
-    // compile-time geometry of k_tile3.hip; grids smaller than a padded tile
-    // fail tiles_ok and take the node-parallel path
     p.NT = 1024;
     int T = 0, P = 0;
     size_t lds = 0;
-    tile3_geometry(dbl, W, &T, &lds, &P);
+    tile3_geometry(dbl, p.W, &T, &lds, &P);
     for (int d = 0; d < 3; ++d) {
       p.T[d] = T;
       p.Pd[d] = P;
@@ -519,16 +524,16 @@ void choose_tiles(nfftm_plan_s& p) {
     p.tileElems = P * P * P;
     p.lds = lds;
     return;
-  } else if (D == 2) {
-    p.NT = 256;
+  }
+  p.NT = 256;
+  if (D == 2) {
     int t = 64;
     while (t > 16 && (double)p.Nt[0] * p.Nt[1] / ((double)t * t) < 256) t /= 2;
-    while (!fit(t, t, 1) && t > 4) t /= 2;
+    while (!set_tile_shape(p, t) && t > 4) t /= 2;
   } else {
-    p.NT = 256;
     int t = dbl ? 2048 : 4096;
     while (t > 256 && (double)p.Nt[0] / t < 256) t /= 2;
-    while (!fit(t, 1, 1) && t > 64) t /= 2;
+    while (!set_tile_shape(p, t) && t > 64) t /= 2;
   }
 }
 
@@ -577,7 +582,8 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
   if (M < 0 || M > 0x7fffffff) return fail(NFFTM_ERR_UNSUPPORTED, "M = %lld out of range", (long long)M);
   if (int rc = check_nodes(k, (i64)D * M)) return rc;
   p.M = M;
-  p.k.assign(k, k + (size_t)D * M);
+  p.gen++;
+  if (k != p.k.data()) p.k.assign(k, k + (size_t)D * M);
   std::vector<double> ks((size_t)D * M);
   std::vector<int> perm;
   std::vector<int4> items;
@@ -796,6 +802,365 @@ int create_impl(nfftm_plan_s& p, const double* k) {
   if (int rc = p.fft->build(p.D, p.Nt, p.dbl())) return rc;
   for (auto& e : p.ev) HIPCHK(hipEventCreate(&e));
   return NFFTM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: tile edge and workgroup size of the gridding kernels, chosen by
+// timing spreading (with its fold) plus interpolation on the plan's own nodes.
+// ---------------------------------------------------------------------------
+struct TuneChoice {
+  int t = 0, NT = 0;
+  double ms = 0;
+};
+
+int time_gridding(nfftm_plan_s& p, void* fh, double* ms) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = NFFTM_OK;
+  float best = 1e30f;
+  for (int rep = 0; rep < 4 && rc == NFFTM_OK; ++rep) {
+    (void)hipEventRecord(e0, p.stream);
+    rc = run_conv(p, true, fh, p.d_grid.p);
+    if (rc == NFFTM_OK) rc = run_conv(p, false, p.d_grid.p, fh);
+    (void)hipEventRecord(e1, p.stream);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(NFFTM_ERR_HIP, "tuning run failed");
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    if (rep > 0) best = std::min(best, t);  // first run warms up
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = best;
+  return rc;
+}
+
+// Candidate shapes: D = 2 tile edges 16..128, D = 1 edges 512..8192, with 256
+// or 512 threads (maxNT caps it); D = 3 keeps its compile-time tile and picks
+// 512 or 1024 threads.  Leaves the fastest applied.
+int tune_plan(nfftm_plan_s& p, int maxNT, std::vector<TuneChoice>* log) {
+  if (!p.tiled || p.M == 0) return NFFTM_OK;
+  DevBuf<char> fh;
+  HIPCHK(fh.alloc((size_t)p.M * p.celem()));
+  HIPCHK(launch_fill0(fh.p, (size_t)p.M * p.celem(), p.stream));
+  std::vector<std::pair<int, int>> cand;
+  if (p.D == 3) {
+    for (int nt : {512, 1024})
+      if (nt <= maxNT) cand.push_back({0, nt});
+  } else {
+    const std::vector<int> ts = p.D == 2 ? std::vector<int>{16, 32, 64, 128}
+                                         : std::vector<int>{512, 1024, 2048, 4096, 8192};
+    for (int t : ts)
+      for (int nt : {256, 512})
+        if (nt <= maxNT) cand.push_back({t, nt});
+  }
+  TuneChoice best;
+  best.ms = 1e30;
+  for (auto& c : cand) {
+    if (p.D != 3) {
+      if (!set_tile_shape(p, c.first) || !tiles_ok(p)) continue;
+      if (int rc = set_nodes(p, p.k.data(), p.M)) return rc;
+    }
+    p.NT = c.second;
+    double ms = 0;
+    if (int rc = time_gridding(p, fh.p, &ms)) return rc;
+    if (log) log->push_back(TuneChoice{c.first, c.second, ms});
+    if (ms < best.ms) best = TuneChoice{c.first, c.second, ms};
+  }
+  if (best.NT == 0) {  // nothing fitted: restore the default shape
+    choose_tiles(p);
+    return set_nodes(p, p.k.data(), p.M);
+  }
+  if (p.D != 3) {
+    set_tile_shape(p, best.t);
+    if (int rc = set_nodes(p, p.k.data(), p.M)) return rc;
+  }
+  p.NT = best.NT;
+  return NFFTM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batches: many plans (and several images per plan's nodes) per launch.
+// ---------------------------------------------------------------------------
+struct nfftm_batch_s {
+  std::vector<nfftm_plan_s*> plans;
+  std::vector<int> gens;  // plan generations the batch was built against
+  int S = 0;  // slots = sum of images per plan
+  bool dbl = false;
+  size_t celem = 8;
+  std::vector<int> slot_plan;
+  std::vector<KArgs> ka;  // host copies (scratch of each slot)
+  std::vector<DArgs> da;
+  DevBuf<KArgs> d_ka;
+  DevBuf<DArgs> d_da;
+  DevBuf<char> grids, scratch;
+  std::vector<void*> grid_ptr;
+  DevBuf<void*> d_grid_ptr, d_tab;  // d_tab: S input then S output pointers of a call
+  void** h_tab = nullptr;           // pinned staging of d_tab
+  hipEvent_t tab_done = nullptr;
+  struct TileGroup {
+    int D = 0, W = 0, NT = 0, nblk = 0, nwork = 0;
+    size_t lds = 0;
+    DevBuf<int2> work;
+  };
+  std::vector<std::unique_ptr<TileGroup>> tg;
+  struct RowWork {
+    DevBuf<int4> work;
+    int nwork = 0, nblk = 0;
+  };
+  RowWork dec, dect, fold[2];
+  struct FFTGroup {
+    std::unique_ptr<FFTStage> fft;
+    void* base = nullptr;
+  };
+  std::vector<FFTGroup> fg;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  bool use_graph = false;
+
+  ~nfftm_batch_s() {
+    for (int i = 0; i < 2; ++i) {
+      if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
+      if (graph[i]) (void)hipGraphDestroy(graph[i]);
+    }
+    if (tab_done) (void)hipEventDestroy(tab_done);
+    if (h_tab) (void)hipHostFree(h_tab);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+namespace {
+
+inline int blocks_for(int nwork, int cap) { return std::max(8, std::min(((nwork + 7) / 8) * 8, cap)); }
+
+// The launches of one direction on stream s, reading the call's data
+// pointers from d_tab (inputs at [0, S), outputs at [S, 2S)).
+int batch_enqueue(nfftm_batch_s& b, bool forward, hipStream_t s) {
+  void* const* tin = b.d_tab.p;
+  void* const* tout = b.d_tab.p + b.S;
+  void* const* grids = b.d_grid_ptr.p;
+  if (forward) {
+    HIPCHK(launch_deconv_batch(b.dbl, b.dec.nblk, s, b.d_da.p, b.dec.work.p, b.dec.nwork, (const void* const*)tin,
+                               grids));
+    for (auto& g : b.fg)
+      if (int rc = g.fft->exec(true, g.base, b.celem, s)) return rc;
+    for (auto& g : b.tg)
+      HIPCHK(launch_tile_batch(g->D, false, b.dbl, g->W, g->NT, g->nblk, g->lds, s, b.d_ka.p, g->work.p, g->nwork,
+                               (const void* const*)grids, tout));
+  } else {
+    for (auto& g : b.tg)
+      HIPCHK(launch_tile_batch(g->D, true, b.dbl, g->W, g->NT, g->nblk, g->lds, s, b.d_ka.p, g->work.p, g->nwork,
+                               (const void* const*)tin, nullptr));
+    for (int d = 0; d < 2; ++d)
+      HIPCHK(launch_fold_batch(d + 1, b.dbl, b.fold[d].nblk, s, b.d_ka.p, b.fold[d].work.p, b.fold[d].nwork, grids));
+    for (auto& g : b.fg)
+      if (int rc = g.fft->exec(false, g.base, b.celem, s)) return rc;
+    HIPCHK(launch_deconv_t_batch(b.dbl, b.dect.nblk, s, b.d_da.p, b.dect.work.p, b.dect.nwork,
+                                 (const void* const*)grids, tout));
+  }
+  return NFFTM_OK;
+}
+
+template <typename T>
+int upload(DevBuf<T>& d, const std::vector<T>& h) {
+  HIPCHK(d.alloc(std::max<size_t>(1, h.size())));
+  if (!h.empty()) HIPCHK(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NFFTM_OK;
+}
+
+int batch_build(nfftm_batch_s& b, const nfftm_plan* plans, const int* ntrans, int nplans, int flags) {
+  b.dbl = plans[0]->dbl();
+  b.celem = plans[0]->celem();
+  for (int i = 0; i < nplans; ++i) {
+    nfftm_plan_s& p = *plans[i];
+    if (p.dbl() != b.dbl) return fail(NFFTM_ERR_ARG, "plans of a batch must share the precision");
+    if (p.D > 2) return fail(NFFTM_ERR_UNSUPPORTED, "plan %d: batches take 1-D and 2-D plans", i);
+    if (!p.tiled) return fail(NFFTM_ERR_UNSUPPORTED, "plan %d: grid too small for the tiled kernels", i);
+    const int K = ntrans ? ntrans[i] : 1;
+    if (K < 1) return fail(NFFTM_ERR_ARG, "ntrans[%d] = %d must be positive", i, K);
+    b.plans.push_back(&p);
+    for (int c = 0; c < K; ++c) b.slot_plan.push_back(i);
+  }
+  b.S = (int)b.slot_plan.size();
+  if (flags & NFFTM_BATCH_AUTOTUNE)
+    for (auto* p : b.plans)
+      if (int rc = tune_plan(*p, 512, nullptr)) return rc;
+  for (auto* p : b.plans) {
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->NT != 256 && p->NT != 512) return fail(NFFTM_ERR_UNSUPPORTED, "batch kernels take 256 or 512 threads");
+    b.gens.push_back(p->gen);
+  }
+
+  // grids grouped by size (one batched rocFFT per group), scratch per slot
+  std::vector<int> order(b.S);
+  for (int s = 0; s < b.S; ++s) order[s] = s;
+  auto key = [&](int s) {
+    const nfftm_plan_s& p = *b.plans[b.slot_plan[s]];
+    return std::make_pair(p.D, std::vector<i64>(p.Nt, p.Nt + p.D));
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
+  std::vector<size_t> goff(b.S), soff(b.S);
+  size_t gtot = 0, stot = 0;
+  for (int s : order) {
+    goff[s] = gtot;
+    gtot += (size_t)b.plans[b.slot_plan[s]]->Q * b.celem;
+  }
+  for (int s = 0; s < b.S; ++s) {
+    const nfftm_plan_s& p = *b.plans[b.slot_plan[s]];
+    soff[s] = stot;
+    stot += (size_t)std::max(1, p.nitems) * p.Pd[0] * p.Pd[1] * p.Pd[2] * b.celem;
+    stot = (stot + 255) & ~(size_t)255;
+  }
+  HIPCHK(b.grids.alloc(gtot));
+  HIPCHK(b.scratch.alloc(stot));
+  b.grid_ptr.resize(b.S);
+  b.ka.resize(b.S);
+  b.da.resize(b.S);
+  for (int s = 0; s < b.S; ++s) {
+    const nfftm_plan_s& p = *b.plans[b.slot_plan[s]];
+    b.grid_ptr[s] = b.grids.p + goff[s];
+    b.ka[s] = p.kargs();
+    b.ka[s].scratch = b.scratch.p + soff[s];
+    b.da[s] = p.dargs();
+  }
+  for (size_t i = 0; i < order.size();) {
+    size_t j = i;
+    while (j < order.size() && key(order[j]) == key(order[i])) ++j;
+    const nfftm_plan_s& p = *b.plans[b.slot_plan[order[i]]];
+    nfftm_batch_s::FFTGroup g;
+    g.fft.reset(new FFTStage());
+    if (int rc = g.fft->build(p.D, p.Nt, b.dbl, j - i)) return rc;
+    g.base = b.grid_ptr[order[i]];
+    b.fg.push_back(std::move(g));
+    i = j;
+  }
+
+  // work lists: tile items per (D, W, NT) group; rows or pairs per slot
+  std::vector<std::vector<int2>> tw;
+  std::vector<int4> dec, dect, fold[2];
+  int dev = 0, cus = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  for (int s = 0; s < b.S; ++s) {
+    const nfftm_plan_s& p = *b.plans[b.slot_plan[s]];
+    int gi = -1;
+    for (size_t g = 0; g < b.tg.size(); ++g)
+      if (b.tg[g]->D == p.D && b.tg[g]->W == p.W && b.tg[g]->NT == p.NT) gi = (int)g;
+    if (gi < 0) {
+      b.tg.emplace_back(new nfftm_batch_s::TileGroup());
+      b.tg.back()->D = p.D;
+      b.tg.back()->W = p.W;
+      b.tg.back()->NT = p.NT;
+      tw.emplace_back();
+      gi = (int)b.tg.size() - 1;
+    }
+    b.tg[gi]->lds = std::max(b.tg[gi]->lds, p.lds);
+    for (int it = 0; it < p.nitems; ++it) tw[gi].push_back(int2{s, it});
+    const i64 npairs = p.Q / 2, CHP = 2048;
+    for (i64 a0 = 0; a0 < npairs; a0 += CHP) dec.push_back(int4{s, (int)a0, (int)std::min(npairs, a0 + CHP), 0});
+    const i64 rows = p.P / p.N[0], CR = std::max<i64>(1, 1024 / p.N[0]);
+    for (i64 r0 = 0; r0 < rows; r0 += CR) dect.push_back(int4{s, (int)r0, (int)std::min(rows, r0 + CR), 0});
+    if (p.D == 2) {
+      const i64 CF = std::max<i64>(1, 1024 / p.Nt[0]);
+      for (i64 r0 = 0; r0 < p.Nt[1]; r0 += CF)
+        fold[1].push_back(int4{s, (int)r0, (int)std::min<i64>(p.Nt[1], r0 + CF), 0});
+    } else {
+      const i64 CF = 4096;
+      for (i64 x0 = 0; x0 < p.Nt[0]; x0 += CF)
+        fold[0].push_back(int4{s, (int)x0, (int)std::min<i64>(p.Nt[0], x0 + CF), 0});
+    }
+  }
+  for (size_t g = 0; g < b.tg.size(); ++g) {
+    auto& G = *b.tg[g];
+    if (int rc = upload(G.work, tw[g])) return rc;
+    G.nwork = (int)tw[g].size();
+    G.nblk = std::min(tile_batch_blocks(G.D, false, b.dbl, G.W, G.NT, G.lds), ((G.nwork + 7) / 8) * 8);
+    G.nblk = std::max(G.nblk, 8);
+  }
+  if (int rc = upload(b.dec.work, dec)) return rc;
+  b.dec.nwork = (int)dec.size();
+  b.dec.nblk = blocks_for(b.dec.nwork, 8 * cus);
+  if (int rc = upload(b.dect.work, dect)) return rc;
+  b.dect.nwork = (int)dect.size();
+  b.dect.nblk = blocks_for(b.dect.nwork, 8 * cus);
+  for (int d = 0; d < 2; ++d) {
+    if (int rc = upload(b.fold[d].work, fold[d])) return rc;
+    b.fold[d].nwork = (int)fold[d].size();
+    b.fold[d].nblk = blocks_for(b.fold[d].nwork, 8 * cus);
+  }
+  if (int rc = upload(b.d_ka, b.ka)) return rc;
+  if (int rc = upload(b.d_da, b.da)) return rc;
+  if (int rc = upload(b.d_grid_ptr, b.grid_ptr)) return rc;
+  HIPCHK(b.d_tab.alloc(2 * (size_t)b.S));
+  HIPCHK(hipHostMalloc((void**)&b.h_tab, 2 * (size_t)b.S * sizeof(void*), hipHostMallocDefault));
+  HIPCHK(hipEventCreate(&b.tab_done));
+  HIPCHK(hipStreamCreate(&b.cap));
+
+  // one eager pass of each direction on zero data (sets kernel attributes,
+  // lets rocFFT finish its lazy set-up), then capture both directions
+  {
+    DevBuf<char> zi, zo;
+    size_t mi = 0, mo = 0;
+    for (int s = 0; s < b.S; ++s) {
+      const nfftm_plan_s& p = *b.plans[b.slot_plan[s]];
+      mi = std::max<size_t>(mi, (size_t)std::max<i64>(p.P, p.M));
+      mo = mi;
+    }
+    HIPCHK(zi.alloc(std::max<size_t>(1, mi) * b.celem));
+    HIPCHK(zo.alloc(std::max<size_t>(1, mo) * b.celem));
+    HIPCHK(launch_fill0(zi.p, mi * b.celem, b.cap));
+    for (int s = 0; s < b.S; ++s) {
+      b.h_tab[s] = zi.p;
+      b.h_tab[b.S + s] = zo.p;
+    }
+    HIPCHK(hipMemcpyAsync(b.d_tab.p, b.h_tab, 2 * (size_t)b.S * sizeof(void*), hipMemcpyHostToDevice, b.cap));
+    for (int dir = 0; dir < 2; ++dir)
+      if (int rc = batch_enqueue(b, dir == 0, b.cap)) return rc;
+    HIPCHK(hipStreamSynchronize(b.cap));
+  }
+  b.use_graph = !(flags & NFFTM_BATCH_NO_GRAPH);
+  for (int dir = 0; dir < 2 && b.use_graph; ++dir) {
+    if (hipStreamBeginCapture(b.cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      b.use_graph = false;
+      break;
+    }
+    const int rc = batch_enqueue(b, dir == 0, b.cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b.cap, &g);
+    if (rc != NFFTM_OK || e != hipSuccess || !g ||
+        hipGraphInstantiate(&b.gexec[dir], g, nullptr, nullptr, 0) != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      b.use_graph = false;
+      (void)hipGetLastError();
+      g_err.clear();
+      break;
+    }
+    b.graph[dir] = g;
+  }
+  return NFFTM_OK;
+}
+
+int batch_run(nfftm_batch_s& b, bool forward, const void* const* in, void* const* out, hipStream_t s) {
+  for (size_t i = 0; i < b.plans.size(); ++i)
+    if (b.plans[i]->gen != b.gens[i])
+      return fail(NFFTM_ERR_ARG, "plan %d of the batch changed its nodes; create the batch again", (int)i);
+  for (int i = 0; i < b.S; ++i)
+    if (!in[i] || !out[i]) return fail(NFFTM_ERR_ARG, "NULL data pointer for slot %d", i);
+  HIPCHK(hipEventSynchronize(b.tab_done));  // the previous call has read the staging table
+  for (int i = 0; i < b.S; ++i) {
+    b.h_tab[i] = const_cast<void*>(in[i]);
+    b.h_tab[b.S + i] = out[i];
+  }
+  HIPCHK(hipMemcpyAsync(b.d_tab.p, b.h_tab, 2 * (size_t)b.S * sizeof(void*), hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(b.tab_done, s));
+  if (b.use_graph) {
+    HIPCHK(hipGraphLaunch(b.gexec[forward ? 0 : 1], s));
+    return NFFTM_OK;
+  }
+  return batch_enqueue(b, forward, s);
 }
 
 }  // namespace
@@ -1025,6 +1390,60 @@ void nfftm_plan_destroy(nfftm_plan p) {
   if (!p) return;
   (void)hipStreamSynchronize(p->stream);
   delete p;
+}
+
+int nfftm_plan_tune(nfftm_plan p) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  return tune_plan(*p, 1024, nullptr);
+}
+
+int nfftm_tile_shape(nfftm_plan p, int* tile, int* threads) {
+  g_err.clear();
+  if (!p || !tile || !threads) return fail(NFFTM_ERR_ARG, "NULL argument");
+  for (int d = 0; d < 3; ++d) tile[d] = p->tiled && d < p->D ? p->T[d] : 0;
+  *threads = p->tiled ? p->NT : 0;
+  return NFFTM_OK;
+}
+
+int nfftm_batch_create(nfftm_batch* out, int nplans, const nfftm_plan* plans, const int* ntrans, int flags) {
+  g_err.clear();
+  if (!out) return fail(NFFTM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (nplans < 1 || !plans) return fail(NFFTM_ERR_ARG, "a batch needs at least one plan");
+  for (int i = 0; i < nplans; ++i)
+    if (!plans[i]) return fail(NFFTM_ERR_ARG, "plan %d is NULL", i);
+  std::unique_ptr<nfftm_batch_s> b(new nfftm_batch_s());
+  if (int rc = batch_build(*b, plans, ntrans, nplans, flags)) return rc;
+  *out = b.release();
+  return NFFTM_OK;
+}
+
+int nfftm_batch_info(nfftm_batch b, int* nslots, int* graph) {
+  g_err.clear();
+  if (!b) return fail(NFFTM_ERR_ARG, "batch is NULL");
+  if (nslots) *nslots = b->S;
+  if (graph) *graph = b->use_graph ? 1 : 0;
+  return NFFTM_OK;
+}
+
+int nfftm_batch_forward(nfftm_batch b, const void* const* f_dev, void* const* fhat_dev, void* hip_stream) {
+  g_err.clear();
+  if (!b || !f_dev || !fhat_dev) return fail(NFFTM_ERR_ARG, "NULL argument");
+  return batch_run(*b, true, f_dev, fhat_dev, (hipStream_t)hip_stream);
+}
+
+int nfftm_batch_adjoint(nfftm_batch b, const void* const* fhat_dev, void* const* f_dev, void* hip_stream) {
+  g_err.clear();
+  if (!b || !f_dev || !fhat_dev) return fail(NFFTM_ERR_ARG, "NULL argument");
+  return batch_run(*b, false, fhat_dev, f_dev, (hipStream_t)hip_stream);
+}
+
+void nfftm_batch_destroy(nfftm_batch b) {
+  if (!b) return;
+  if (b->tab_done) (void)hipEventSynchronize(b->tab_done);
+  (void)hipDeviceSynchronize();
+  delete b;
 }
 
 }  // extern "C"

@@ -429,6 +429,21 @@ class NFFTPlan:
                                           ctypes.c_void_p(kb.data_ptr())))
         return kb.t().flip(0)
 
+    def tune(self):
+        """Autotune the tile edge and workgroup size of the gridding kernels on
+        this plan's nodes (nfftm_plan_tune); returns the chosen shape."""
+        torch.cuda.current_stream().synchronize()
+        self._lib.nfftm_set_stream(self._handle, _stream())
+        _check(self._lib.nfftm_plan_tune(self._handle))
+        return self.tile_shape()
+
+    def tile_shape(self):
+        """(tile edge per dimension, workgroup size) of the gridding kernels."""
+        t = (ctypes.c_int * 3)()
+        nt = ctypes.c_int()
+        _check(self._lib.nfftm_tile_shape(self._handle, t, ctypes.byref(nt)))
+        return tuple(t[: self.D])[::-1], nt.value
+
     # optional interface (AbstractNFFTs/src/interface.jl:217-243) ---------------
     def _stage(self, fn, inp, in_shape, out, out_shape, names):
         real_in = isinstance(inp, torch.Tensor) and not inp.is_complex() or (
@@ -473,6 +488,72 @@ class NFFTPlan:
         self._lib.nfftm_set_stream(self._handle, _stream())
         _check(self._lib.nfftm_fft(self._handle, ctypes.c_void_p(g.data_ptr()), int(direction)))
         return g
+
+
+class BatchPlan:
+    """Many transforms per launch (nfftm_batch_*): plans of one precision and
+    D <= 2 with different N, M and nodes, and ``ntrans[i]`` images per plan
+    that share its nodes.  Slots are (plan i, image c) in order of i then c;
+    ``forward``/``adjoint`` take and return one tensor per slot and compute
+    exactly what ``plans[i] * f`` / ``plans[i].H * fhat`` compute, with every
+    stage of all slots in one launch and each direction replayed from a HIP
+    graph.  The plans must stay alive and keep their nodes while the batch is
+    used."""
+
+    def __init__(self, plans, ntrans=None, *, autotune=True, graph=True):
+        self._lib = _L.load()
+        self.plans = list(plans)
+        n = len(self.plans)
+        if n == 0:
+            raise ArgumentError("a batch needs at least one plan")
+        self.ntrans = [1] * n if ntrans is None else [int(x) for x in ntrans]
+        if len(self.ntrans) != n:
+            raise DimensionMismatch("ntrans needs one entry per plan")
+        self.slots = [(i, c) for i in range(n) for c in range(self.ntrans[i])]
+        handles = (ctypes.c_void_p * n)(*[p._handle.value for p in self.plans])
+        nt = (ctypes.c_int * n)(*self.ntrans)
+        flags = (_L.BATCH_AUTOTUNE if autotune else 0) | (0 if graph else _L.BATCH_NO_GRAPH)
+        torch.cuda.current_stream().synchronize()
+        self._handle = ctypes.c_void_p()
+        _check(self._lib.nfftm_batch_create(ctypes.byref(self._handle), n, handles, nt, flags))
+        ns, g = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.nfftm_batch_info(self._handle, ctypes.byref(ns), ctypes.byref(g)))
+        self.nslots, self.graph = ns.value, bool(g.value)
+        self.dtype = self.plans[0].dtype
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                self._lib.nfftm_batch_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    def _call(self, fn, xs, in_shape, out_shape, outs):
+        if len(xs) != self.nslots:
+            raise DimensionMismatch(f"expected {self.nslots} inputs (one per slot), got {len(xs)}")
+        ins = [self.plans[i]._prep(x, in_shape(self.plans[i]), "input")
+               for x, (i, _) in zip(xs, self.slots)]
+        if outs is None:
+            outs = [torch.empty(out_shape(self.plans[i]), dtype=self.dtype, device=_device())
+                    for (i, _) in self.slots]
+        for o, (i, _) in zip(outs, self.slots):
+            if (not isinstance(o, torch.Tensor) or o.dtype != self.dtype or not o.is_contiguous()
+                    or tuple(o.shape) != tuple(out_shape(self.plans[i]))):
+                raise DimensionMismatch("outputs must be contiguous device tensors of the plan shapes")
+        pin = (ctypes.c_void_p * self.nslots)(*[x.data_ptr() for x in ins])
+        pout = (ctypes.c_void_p * self.nslots)(*[o.data_ptr() for o in outs])
+        _check(fn(self._handle, pin, pout, _stream()))
+        return outs
+
+    def forward(self, fs, outs=None):
+        """fHat of every slot: plans[i] * fs[s] for slot s = (i, c)."""
+        return self._call(self._lib.nfftm_batch_forward, fs, lambda p: p.N, lambda p: (p.J,), outs)
+
+    def adjoint(self, fhats, outs=None):
+        """f of every slot: adjoint(plans[i]) * fhats[s]."""
+        return self._call(self._lib.nfftm_batch_adjoint, fhats, lambda p: (p.J,), lambda p: p.N, outs)
 
 
 class AdjointPlan:

@@ -16,6 +16,7 @@ F32, F64, MIXED = 0, 1, 2
 FULL, TENSOR, LINEAR, POLYNOMIAL = 1, 2, 3, 4
 KAISER_BESSEL = 0
 FLAG_BLOCKING, FLAG_TIMING = 1, 2
+BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
 
 # every symbol include/nfftm.h declares
 EXPORTS = (
@@ -23,7 +24,9 @@ EXPORTS = (
     "nfftm_deconvolve", "nfftm_deconvolve_transpose", "nfftm_convolve",
     "nfftm_convolve_transpose", "nfftm_fft", "nfftm_sizes", "nfftm_grid",
     "nfftm_timing", "nfftm_set_timing", "nfftm_profile", "nfftm_profile_read",
-    "nfftm_set_stream", "nfftm_plan_copy", "nfftm_grad_nodes",
+    "nfftm_set_stream", "nfftm_plan_copy", "nfftm_grad_nodes", "nfftm_plan_tune",
+    "nfftm_tile_shape", "nfftm_batch_create", "nfftm_batch_info", "nfftm_batch_forward",
+    "nfftm_batch_adjoint", "nfftm_batch_destroy",
     "nfftm_plan_destroy", "nfftm_last_error", "nfftm_version",
 )
 
@@ -49,6 +52,14 @@ def load():
     L.nfftm_plan_create.argtypes = [ctypes.POINTER(vp), i, i, pi64, i64, pd, i, d, i, i, i, vp]
     L.nfftm_nodes.argtypes = [vp, pd, i64]
     L.nfftm_grad_nodes.argtypes = [vp, vp, vp, vp]
+    L.nfftm_plan_tune.argtypes = [vp]
+    L.nfftm_tile_shape.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.nfftm_batch_create.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(vp), ctypes.POINTER(i), i]
+    L.nfftm_batch_info.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.nfftm_batch_forward.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), vp]
+    L.nfftm_batch_adjoint.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), vp]
+    L.nfftm_batch_destroy.argtypes = [vp]
+    L.nfftm_batch_destroy.restype = None
     for name in ("nfftm_forward", "nfftm_adjoint", "nfftm_deconvolve",
                  "nfftm_deconvolve_transpose", "nfftm_convolve", "nfftm_convolve_transpose"):
         getattr(L, name).argtypes = [vp, vp, vp]
@@ -68,7 +79,8 @@ def load():
     L.nfftm_version.restype = ctypes.c_char_p
     L.nfftm_version.argtypes = []
     for name in EXPORTS:
-        if name not in ("nfftm_plan_destroy", "nfftm_last_error", "nfftm_version"):
+        if name not in ("nfftm_plan_destroy", "nfftm_last_error", "nfftm_version",
+                        "nfftm_batch_destroy"):
             getattr(L, name).restype = i
     _lib = L
     return L

@@ -78,6 +78,13 @@ def test_unsupported_options():
 
 def test_null_plan_calls_fail_cleanly():
     L = _lib.load()
+    b = ctypes.c_void_p()
+    assert L.nfftm_batch_create(ctypes.byref(b), 0, None, None, 0) == _lib.ERR_ARG
+    plans = (ctypes.c_void_p * 2)(None, None)
+    assert L.nfftm_batch_create(ctypes.byref(b), 2, plans, None, 0) == _lib.ERR_ARG
+    assert L.nfftm_batch_forward(None, None, None, None) == _lib.ERR_ARG
+    assert L.nfftm_plan_tune(None) == _lib.ERR_ARG
+    L.nfftm_batch_destroy(None)
     for name in ("nfftm_forward", "nfftm_adjoint", "nfftm_convolve", "nfftm_convolve_transpose",
                  "nfftm_deconvolve", "nfftm_deconvolve_transpose"):
         assert getattr(L, name)(None, None, None) == _lib.ERR_ARG
