@@ -287,6 +287,31 @@ hipError_t launch_deconv_t_batch(bool dbl, int nblk, hipStream_t s, const DArgs*
   return hipGetLastError();
 }
 
+template <typename R, int D>
+static hipError_t records_w(int W, const KArgs& a, char* rec, hipStream_t s) {
+  const dim3 g((unsigned)a.nitems), b(256);
+  switch (W) {
+#define NFFTM_REC(WW) \
+  case WW:            \
+    hipLaunchKernelGGL((k_node_records<R, D, WW>), g, b, 0, s, a, rec); \
+    break;
+    NFFTM_REC(2) NFFTM_REC(4) NFFTM_REC(6) NFFTM_REC(8) NFFTM_REC(10) NFFTM_REC(12) NFFTM_REC(14) NFFTM_REC(16)
+#undef NFFTM_REC
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_node_records(int D, bool dbl, int W, const KArgs& a, void* rec, hipStream_t s) {
+  if (a.nitems == 0) return hipSuccess;
+  char* r = (char*)rec;
+  if (D == 1) return dbl ? records_w<double, 1>(W, a, r, s) : records_w<float, 1>(W, a, r, s);
+  if (D == 2) return dbl ? records_w<double, 2>(W, a, r, s) : records_w<float, 2>(W, a, r, s);
+  if (D == 3) return dbl ? records_w<double, 3>(W, a, r, s) : records_w<float, 3>(W, a, r, s);
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s) {
   const long long n16 = (long long)(bytes / 16);
   if (n16 > 0) {

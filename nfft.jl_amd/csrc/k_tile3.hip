@@ -10,7 +10,7 @@
 namespace nfftm {
 
 // Tile edge per precision and stencil width W = 2m, the largest that keeps
-// the padded tile (T + W - 1)^3 plus the per-batch node records within
+// the padded tile (T + W - 1)^3 plus one batch of node records within
 // 160 KiB of LDS.
 template <typename R, int W>
 struct Geo3 {
@@ -22,17 +22,14 @@ struct Geo3 {
   static constexpr int CELLS = SXY * P;                 // elements of one LDS plane
   static constexpr int PAD = P * P * P;                 // dense padded cells
   static constexpr size_t TILE_BYTES = 2 * (size_t)CELLS * sizeof(R);
-  static constexpr size_t rec_bytes(int nb) {
-    return (size_t)nb * 3 * W * sizeof(R) + (size_t)nb * 2 * sizeof(R) + (size_t)nb * 8;
-  }
-  static constexpr int NB = TILE_BYTES + rec_bytes(256) + 64 <= 163840   ? 256
-                            : TILE_BYTES + rec_bytes(128) + 64 <= 163840 ? 128
-                                                                         : 64;
-  static constexpr size_t WTS_OFF = TILE_BYTES;
-  static constexpr size_t FV_OFF = (WTS_OFF + (size_t)NB * 3 * W * sizeof(R) + 15) & ~(size_t)15;
-  static constexpr size_t LOC_OFF = FV_OFF + (size_t)NB * 2 * sizeof(R);
-  static constexpr size_t PJ_OFF = LOC_OFF + (size_t)NB * 4;
-  static constexpr size_t LDS = PJ_OFF + (size_t)NB * 4;
+  static constexpr int RB = NodeRec<R, 3, W>::BYTES;
+  static constexpr size_t REC_OFF = (TILE_BYTES + 15) & ~(size_t)15;
+  static constexpr size_t batch_bytes(int nb) { return (size_t)nb * RB + (size_t)nb * 2 * sizeof(R); }
+  static constexpr int NB = REC_OFF + batch_bytes(256) <= 163840   ? 256
+                            : REC_OFF + batch_bytes(128) <= 163840 ? 128
+                                                                   : 64;
+  static constexpr size_t FV_OFF = REC_OFF + (size_t)NB * RB;
+  static constexpr size_t LDS = FV_OFF + (size_t)NB * 2 * sizeof(R);
   static_assert(LDS <= 163840, "3-D tile does not fit LDS");
 };
 
@@ -50,45 +47,23 @@ __device__ __forceinline__ R wave_total(R v) {
   return v;
 }
 
-// Phase A for D = 3: node records of nodes [b0, b0 + nb) of the item.
-template <typename R, int W, int NT, bool SPREAD>
-__device__ __forceinline__ void phase_a3(const KArgs& a, const typename cpx<R>::T* __restrict__ fhat, int b0,
-                                         int nb, const int* t0, R* wts, typename cpx<R>::T* fv, int* loc,
-                                         int* pj, R fscale) {
-  for (int q = threadIdx.x; q < nb * 3; q += NT) {
-    const int n = q / 3, d = q - n * 3;
-    const int j = b0 + n;
-    R w[W];
-    const int cw = node_weights<R, W>(a, d, a.ks[(size_t)d * a.M + j], w);
-#pragma unroll
-    for (int l = 0; l < W; ++l) wts[(n * 3 + d) * W + l] = w[l];
-    // packed local offsets: 8 bits per dimension (T + W <= 32)
-    atomicOr(&loc[n], (cw - t0[d]) << (8 * d));
-    if (d == 0) {
-      const int jj = a.perm[j];
-      pj[n] = jj;
-      if (SPREAD) {
-        const typename cpx<R>::T v = fhat[jj];
-        fv[n] = typename cpx<R>::T{v.x * fscale * (R)a.oscale, v.y * fscale * (R)a.oscale};
-      }
-    }
-  }
-}
-
+// Interpolation: the padded tile is staged once; each batch of node records
+// is copied to LDS; 16 lanes per node, 4 nodes per wavefront, each lane
+// walking ceil(W^2 / 16) (x, y) columns of the footprint along z, and a
+// 16-lane DPP reduction per node.
 template <typename R, int W, int NT>
 __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
                                                 typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
   using G = Geo3<R, W>;
   constexpr int NW = NT / 64;
+  constexpr int NCOL = (W * W + 15) / 16;
   extern __shared__ __align__(16) char smem[];
   const int item = xcd_item(a.nitems);
   if (item >= a.nitems) return;
   const int4 it = a.items[item];
   C* tile = (C*)smem;
-  R* wts = (R*)(smem + G::WTS_OFF);
-  int* loc = (int*)(smem + G::LOC_OFF);
-  int* pj = (int*)(smem + G::PJ_OFF);
+  char* lrec = smem + G::REC_OFF;
   int t0[3];
   {
     const int tx = it.x % a.ntiles[0], q = it.x / a.ntiles[0];
@@ -114,55 +89,54 @@ __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::
   }
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int quarter = lane >> 4, r16 = lane & 15;
   const R osc = (R)a.oscale;
   for (int b0 = it.y; b0 < it.z; b0 += G::NB) {
     const int nb = min(G::NB, it.z - b0);
     __syncthreads();
-    for (int q = threadIdx.x; q < nb; q += NT) loc[q] = 0;
+    phase_a<R, 3, W, NT, false>(a, nullptr, b0, nb, lrec, nullptr, (R)1);
     __syncthreads();
-    phase_a3<R, W, NT, false>(a, nullptr, b0, nb, t0, wts, nullptr, loc, pj, (R)1);
-    __syncthreads();
-    for (int n = 2 * wave; n < nb; n += 2 * NW) {
-      const bool two = n + 1 < nb;
-      R acc[2][2] = {{0, 0}, {0, 0}};
+    for (int n0 = 4 * wave; n0 < nb; n0 += 4 * NW) {
+      const int n = n0 + quarter;
+      const bool live = n < nb;
+      const int nn = live ? n : n0;
+      const R* w = rec_w<R, 3, W>(lrec, nn);
+      R w3[W];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int nn = two ? n + u : n;
-        const R* w = wts + nn * 3 * W;
-        R w3[W];
+      for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
+      const unsigned pk = rec_loc<R, 3, W>(lrec, nn);
+      const int bx = pk & 1023, by = (pk >> 10) & 1023, bz = pk >> 20;
+      const C* base0 = tile + bx + G::SX * by + G::SXY * bz;
+      R ar = 0, ai = 0;
 #pragma unroll
-        for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
-        const int pk = loc[nn];
-        const int bx = pk & 255, by = (pk >> 8) & 255, bz = (pk >> 16) & 255;
+      for (int c = 0; c < NCOL; ++c) {
+        const int t = r16 + 16 * c;
+        if (t < W * W) {
+          const int ly = t / W, lx = t - ly * W;
+          const C* col = base0 + lx + G::SX * ly;
+          R sr = 0, si = 0;
 #pragma unroll
-        for (int t0l = 0; t0l < W * W; t0l += 64) {
-          const int t = t0l + lane;
-          if (t < W * W) {
-            const int ly = t / W, lx = t - ly * W;
-            const C* base = tile + (bx + lx) + G::SX * (by + ly) + G::SXY * bz;
-            R sr = 0, si = 0;
-#pragma unroll
-            for (int lz = 0; lz < W; ++lz) {
-              const C v = base[G::SXY * lz];
-              sr = fma(w3[lz], v.x, sr);
-              si = fma(w3[lz], v.y, si);
-            }
-            const R wxy = w[lx] * w[W + ly];
-            acc[u][0] = fma(wxy, sr, acc[u][0]);
-            acc[u][1] = fma(wxy, si, acc[u][1]);
+          for (int lz = 0; lz < W; ++lz) {
+            const C v = col[G::SXY * lz];
+            sr = fma(w3[lz], v.x, sr);
+            si = fma(w3[lz], v.y, si);
           }
+          const R wxy = w[lx] * w[W + ly];
+          ar = fma(wxy, sr, ar);
+          ai = fma(wxy, si, ai);
         }
       }
-      const R s0 = wave_total(acc[0][0]), s1 = wave_total(acc[0][1]);
-      const R s2 = wave_total(acc[1][0]), s3 = wave_total(acc[1][1]);
-      if (lane == 63) {
-        fhat[pj[n]] = C{s0 * osc, s1 * osc};
-        if (two) fhat[pj[n + 1]] = C{s2 * osc, s3 * osc};
-      }
+      ar = group_sum<16>(ar);
+      ai = group_sum<16>(ai);
+      if (live && r16 == 0) fhat[rec_pj<R, 3, W>(lrec, n)] = C{ar * osc, ai * osc};
     }
   }
 }
 
+// Spreading: lanes over the (x, y) columns of a node's footprint, walking z,
+// one LDS atomic per tap (FP32: packed 32+32-bit fixed point with a per-item
+// scale from a strict bound; FP64: two FP64 planes); the padded tile goes to
+// the item's scratch slot.
 template <typename R, int W, int NT>
 __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
@@ -173,17 +147,8 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
   if (item >= a.nitems) return;
   const int4 it = a.items[item];
   char* tile = smem;
-  R* wts = (R*)(smem + G::WTS_OFF);
+  char* lrec = smem + G::REC_OFF;
   C* fv = (C*)(smem + G::FV_OFF);
-  int* loc = (int*)(smem + G::LOC_OFF);
-  int* pj = (int*)(smem + G::PJ_OFF);
-  int t0[3];
-  {
-    const int tx = it.x % a.ntiles[0], q = it.x / a.ntiles[0];
-    t0[0] = tx * G::T;
-    t0[1] = (q % a.ntiles[1]) * G::T;
-    t0[2] = (q / a.ntiles[1]) * G::T;
-  }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 
   // FP32: fixed-point scale s = 2^(30 - e), B = sum_j |f_j|_inf * wmaxD < 2^e
@@ -195,7 +160,7 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
       part += fmax(fabs(v.x), fabs(v.y));
     }
     part = wave_total(part);
-    R* red = wts;
+    R* red = (R*)lrec;
     if (lane == 63) red[wave] = part;
     __syncthreads();
     R B = 0;
@@ -211,18 +176,16 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
   for (int b0 = it.y; b0 < it.z; b0 += G::NB) {
     const int nb = min(G::NB, it.z - b0);
     __syncthreads();
-    for (int q = threadIdx.x; q < nb; q += NT) loc[q] = 0;
-    __syncthreads();
-    phase_a3<R, W, NT, true>(a, fhat, b0, nb, t0, wts, fv, loc, pj, fscale);
+    phase_a<R, 3, W, NT, true>(a, fhat, b0, nb, lrec, fv, fscale * (R)a.oscale);
     __syncthreads();
     for (int n = wave; n < nb; n += NW) {
       const C f = fv[n];
-      const R* w = wts + n * 3 * W;
+      const R* w = rec_w<R, 3, W>(lrec, n);
       R w3[W];
 #pragma unroll
       for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
-      const int pk = loc[n];
-      const int bx = pk & 255, by = (pk >> 8) & 255, bz = (pk >> 16) & 255;
+      const unsigned pk = rec_loc<R, 3, W>(lrec, n);
+      const int bx = pk & 1023, by = (pk >> 10) & 1023, bz = pk >> 20;
 #pragma unroll
       for (int t0l = 0; t0l < W * W; t0l += 64) {
         const int t = t0l + lane;

@@ -103,6 +103,8 @@ struct KArgs {
   const int* perm;    // sorted position -> caller's node index
   const int4* items;  // {tile, begin, end, 0}
   int nitems;
+  const char* rec;    // node records in sorted order (NodeRec layout), tiled plans
+  int recBytes;
   // spreading
   double wmaxD;        // bound of any product of D window values
   void* scratch;       // nitems padded tiles (dense, px fastest)
@@ -251,36 +253,42 @@ __device__ __forceinline__ void gbl_add(R* p, R v) {
 }
 
 // ---------------------------------------------------------------------------
-// LDS layout of the tile kernels (dynamic shared memory):
-//   [tile: 2 * tileElems R]  [wts: NB * D * W R]  [fv: NB complex]
-//   [loc: NB * D int] [pj: NB int]
+// Node records and the LDS layout of the tile kernels
 // ---------------------------------------------------------------------------
+// Node record, one per node in tile order, built once per node set: the
+// window weights of every dimension (D x W values of R), the node's cell
+// relative to its tile origin packed in 32 / D bits per dimension, and the
+// caller's index of the node.  16-byte aligned for block copies.
+template <typename R, int D, int W>
+struct NodeRec {
+  static constexpr int WBYTES = D * W * (int)sizeof(R);
+  static constexpr int BYTES = (WBYTES + 8 + 15) & ~15;
+  static constexpr int LOCBITS = 32 / D;
+};
+__host__ inline int node_rec_bytes(int D, size_t sizeofR, int W) {
+  return (int)((D * W * sizeofR + 8 + 15) & ~(size_t)15);
+}
+
+// LDS layout of the D <= 2 tile kernels: tile, node records of one batch,
+// node values of one batch (spreading).
 template <typename R, int D, int W>
 struct TileLayout {
   // the spreading tile of D <= 2 accumulates in FP64 in both precisions
   static constexpr size_t ACC = D <= 2 ? sizeof(double) : sizeof(R);
-  static __host__ __device__ size_t wts_off(int tileElems) { return 2 * (size_t)tileElems * ACC; }
+  static __host__ __device__ size_t rec_off(int tileElems) { return (2 * (size_t)tileElems * ACC + 15) & ~(size_t)15; }
   static __host__ __device__ size_t fv_off(int tileElems, int NB) {
-    size_t o = wts_off(tileElems) + (size_t)NB * D * W * sizeof(R);
-    return (o + 15) & ~(size_t)15;
-  }
-  static __host__ __device__ size_t loc_off(int tileElems, int NB) {
-    return fv_off(tileElems, NB) + (size_t)NB * 2 * sizeof(R);
-  }
-  static __host__ __device__ size_t pj_off(int tileElems, int NB) {
-    return loc_off(tileElems, NB) + (size_t)NB * D * sizeof(int);
+    return rec_off(tileElems) + (size_t)NB * NodeRec<R, D, W>::BYTES;
   }
   static __host__ __device__ size_t bytes(int tileElems, int NB) {
-    return pj_off(tileElems, NB) + (size_t)NB * sizeof(int);
+    return fv_off(tileElems, NB) + (size_t)NB * 2 * sizeof(R);
   }
 };
 
 // The same layout size for a runtime stencil width (host side).
 __host__ inline size_t tile_layout_bytes(int D, size_t sizeofR, int W, int tileElems, int NB) {
   const size_t acc = D <= 2 ? sizeof(double) : sizeofR;
-  size_t o = 2 * (size_t)tileElems * acc + (size_t)NB * D * W * sizeofR;
-  o = (o + 15) & ~(size_t)15;
-  return o + (size_t)NB * 2 * sizeofR + (size_t)NB * D * sizeof(int) + (size_t)NB * sizeof(int);
+  const size_t o = (2 * (size_t)tileElems * acc + 15) & ~(size_t)15;
+  return o + (size_t)NB * node_rec_bytes(D, sizeofR, W) + (size_t)NB * 2 * sizeofR;
 }
 
 // Nodes per phase-A batch.
@@ -298,30 +306,70 @@ __device__ __forceinline__ void tile_origin(const KArgs& a, int D, int tile, int
   }
 }
 
-// Phase A: weights, local offsets, values and output indices of a batch of
-// nodes [b0, b0 + nb) into LDS.  One thread per (node, dim).
-template <typename R, int D, int W, int NT, bool SPREAD>
-__device__ __forceinline__ void phase_a(const KArgs& a, const typename cpx<R>::T* __restrict__ fhat,
-                                        int b0, int nb, const int* t0, R* wts,
-                                        typename cpx<R>::T* fv, int* loc, int* pj) {
-  for (int q = threadIdx.x; q < nb * D; q += NT) {
-    const int n = q / D, d = q - n * D;
-    const int j = b0 + n;
-    const double ksv = a.ks[(size_t)d * a.M + j];
-    R w[W];
-    const int cw = node_weights<R, W>(a, d, ksv, w);
-#pragma unroll
-    for (int l = 0; l < W; ++l) wts[(n * D + d) * W + l] = w[l];
-    loc[n * D + d] = cw - t0[d];
-    if (d == 0) {
-      const int jj = a.perm[j];
-      pj[n] = jj;
-      if (SPREAD) {
-        const typename cpx<R>::T v = fhat[jj];
-        fv[n] = typename cpx<R>::T{v.x * (R)a.oscale, v.y * (R)a.oscale};
-      }
+// Node records of all nodes of one work item (one workgroup per item, one
+// thread per node): weights from node_weights, the cell relative to the
+// tile origin, the caller's index.
+template <typename R, int D, int W>
+__global__ __launch_bounds__(256) void k_node_records(KArgs a, char* __restrict__ rec) {
+  using N = NodeRec<R, D, W>;
+  const int4 it = a.items[blockIdx.x];
+  int t0[3] = {0, 0, 0};
+  {
+    int r = it.x;
+    for (int d = 0; d < D; ++d) {
+      const int td = r % a.ntiles[d];
+      r /= a.ntiles[d];
+      t0[d] = td * a.T[d];
     }
   }
+  for (int j = it.y + threadIdx.x; j < it.z; j += 256) {
+    char* out = rec + (size_t)j * N::BYTES;
+    R* w = (R*)out;
+    unsigned pk = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      R wd[W];
+      const int cw = node_weights<R, W>(a, d, a.ks[(size_t)d * a.M + j], wd);
+#pragma unroll
+      for (int l = 0; l < W; ++l) w[d * W + l] = wd[l];
+      pk |= (unsigned)(cw - t0[d]) << (N::LOCBITS * d);
+    }
+    ((int*)(out + N::WBYTES))[0] = (int)pk;
+    ((int*)(out + N::WBYTES))[1] = a.perm[j];
+  }
+}
+
+// Phase A: copy the records of nodes [b0, b0 + nb) into LDS (16-byte
+// blocks); for spreading also gather their values fhat[pj] * scale.
+template <typename R, int D, int W, int NT, bool SPREAD>
+__device__ __forceinline__ void phase_a(const KArgs& a, const typename cpx<R>::T* __restrict__ fhat, int b0, int nb,
+                                        char* lrec, typename cpx<R>::T* fv, R scale) {
+  using N = NodeRec<R, D, W>;
+  const uint4* src = (const uint4*)(a.rec + (size_t)b0 * N::BYTES);
+  uint4* dst = (uint4*)lrec;
+  const int n16 = nb * (N::BYTES / 16);
+  for (int i = threadIdx.x; i < n16; i += NT) dst[i] = src[i];
+  if (SPREAD) {
+    for (int n = threadIdx.x; n < nb; n += NT) {
+      const int jj = ((const int*)(a.rec + (size_t)(b0 + n) * N::BYTES + N::WBYTES))[1];
+      const typename cpx<R>::T v = fhat[jj];
+      fv[n] = typename cpx<R>::T{v.x * scale, v.y * scale};
+    }
+  }
+}
+
+// Fields of a record in LDS.
+template <typename R, int D, int W>
+__device__ __forceinline__ const R* rec_w(const char* lrec, int n) {
+  return (const R*)(lrec + n * NodeRec<R, D, W>::BYTES);
+}
+template <typename R, int D, int W>
+__device__ __forceinline__ unsigned rec_loc(const char* lrec, int n) {
+  return ((const unsigned*)(lrec + n * NodeRec<R, D, W>::BYTES + NodeRec<R, D, W>::WBYTES))[0];
+}
+template <typename R, int D, int W>
+__device__ __forceinline__ int rec_pj(const char* lrec, int n) {
+  return ((const int*)(lrec + n * NodeRec<R, D, W>::BYTES + NodeRec<R, D, W>::WBYTES))[1];
 }
 
 // ---------------------------------------------------------------------------
@@ -362,97 +410,54 @@ struct SpreadCell<double> {  // two FP64 planes
   }
 };
 
-// One work item (<= CH nodes of one tile) of spreading: the padded tile is
-// accumulated in LDS and written to the item's slot of `scratch`.
+// One work item (<= CH nodes of one tile, D <= 2) of spreading: the padded
+// tile is accumulated in LDS -- FP64 adds of the data-precision products, one
+// rounding per term -- and written to the item's slot of `scratch`.
 template <typename R, int D, int W, int NT>
 __device__ __forceinline__ void spread_tile_item(const KArgs& a, int item,
                                                  const typename cpx<R>::T* __restrict__ fhat,
                                                  typename cpx<R>::T* __restrict__ scratch) {
+  static_assert(D <= 2, "3-D spreading lives in k_tile3.hip");
   using C = typename cpx<R>::T;
   using L = TileLayout<R, D, W>;
-  // Accumulator: FP64 LDS adds for D <= 2 (FP32 products, one rounding per
-  // term, no range bound); packed fixed point for FP32 in 3-D.
-  using A = typename std::conditional<(D <= 2), double, R>::type;
+  using A = double;
   constexpr int NB = Batch<D, R>::NB;
   constexpr int NW = NT / 64;
+  constexpr unsigned LM = NodeRec<R, D, W>::LOCBITS >= 32 ? 0xFFFFFFFFu : (1u << NodeRec<R, D, W>::LOCBITS) - 1u;
   extern __shared__ __align__(16) char smem[];
   const int4 it = a.items[item];
   char* tile = smem;
-  R* wts = (R*)(smem + L::wts_off(a.tileElems));
+  char* lrec = smem + L::rec_off(a.tileElems);
   C* fv = (C*)(smem + L::fv_off(a.tileElems, NB));
-  int* loc = (int*)(smem + L::loc_off(a.tileElems, NB));
-  int* pj = (int*)(smem + L::pj_off(a.tileElems, NB));
-  int t0[3] = {0, 0, 0};
-  tile_origin(a, D, it.x, t0);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-
-  // FP32: fixed-point scale s = 2^(30 - ceil(log2 B)), B = sum_j |f_j|_inf * wmaxD
-  R fscale = 1, inv = 1;
-  if (sizeof(A) == 4) {
-    R part = 0;
-    for (int j = it.y + threadIdx.x; j < it.z; j += NT) {
-      const C v = fhat[a.perm[j]];
-      part += fmax(fabs(v.x), fabs(v.y));
-    }
-    part = group_sum<64>(part);
-    R* red = wts;  // scratch before phase A
-    if (lane == 0) red[wave] = part;
-    __syncthreads();
-    R B = 0;
-    for (int w = 0; w < NW; ++w) B += red[w];
-    B = (R)((double)B * a.oscale * a.wmaxD * 1.0000001);
-    int e = 0;
-    if (B > 0) (void)frexp((double)B, &e);  // B < 2^e
-    fscale = (R)ldexp(1.0, 30 - e);
-    inv = (R)ldexp(1.0, e - 30);
-  }
 
   for (int i = threadIdx.x; i < 2 * a.tileElems; i += NT) ((A*)tile)[i] = 0;
 
+  constexpr int G = LanesPerNode<W>::value;
+  constexpr int NPW = 64 / G;
+  const int slot = lane / G, lx = lane & (G - 1);
   for (int b0 = it.y; b0 < it.z; b0 += NB) {
     const int nb = min(NB, it.z - b0);
     __syncthreads();
-    phase_a<R, D, W, NT, true>(a, fhat, b0, nb, t0, wts, fv, loc, pj);
+    phase_a<R, D, W, NT, true>(a, fhat, b0, nb, lrec, fv, (R)a.oscale);
     __syncthreads();
-    if constexpr (D == 3) {
-      for (int n = wave; n < nb; n += NW) {
+    for (int n0 = wave * NPW; n0 < nb; n0 += NW * NPW) {
+      const int n = n0 + slot;
+      if (n < nb && lx < W) {
         const C f = fv[n];
-        const R* w = wts + n * 3 * W;
-        R w3[W];
+        const R* w = rec_w<R, D, W>(lrec, n);
+        const unsigned pk = rec_loc<R, D, W>(lrec, n);
+        const R w1 = w[lx];
+        const R vr = f.x * w1, vi = f.y * w1;
+        if constexpr (D == 2) {
+          const int base = ((int)(pk & LM) + lx) + a.Sx * (int)(pk >> 16);
 #pragma unroll
-        for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
-        const int bx = loc[n * 3], by = loc[n * 3 + 1], bz = loc[n * 3 + 2];
-        for (int t = lane; t < W * W; t += 64) {
-          const int ly = t / W, lx = t - ly * W;
-          const R wxy = w[lx] * w[W + ly] * fscale;
-          const R vr = f.x * wxy, vi = f.y * wxy;
-          const int base = (bx + lx) + a.Sx * (by + ly) + a.Sxy * bz;
-#pragma unroll
-          for (int lz = 0; lz < W; ++lz)
-            SpreadCell<A>::add(tile, a.tileElems, base + a.Sxy * lz, vr * w3[lz], vi * w3[lz]);
-        }
-      }
-    } else {
-      constexpr int G = LanesPerNode<W>::value;
-      constexpr int NPW = 64 / G;
-      const int slot = lane / G, lx = lane & (G - 1);
-      for (int n0 = wave * NPW; n0 < nb; n0 += NW * NPW) {
-        const int n = n0 + slot;
-        if (n < nb && lx < W) {
-          const C f = fv[n];
-          const R* w = wts + n * D * W;
-          const R w1 = w[lx] * fscale;
-          const R vr = f.x * w1, vi = f.y * w1;
-          if constexpr (D == 2) {
-            const int base = (loc[n * 2] + lx) + a.Sx * loc[n * 2 + 1];
-#pragma unroll
-            for (int ly = 0; ly < W; ++ly) {
-              const R w2 = w[W + ly];
-              SpreadCell<A>::add(tile, a.tileElems, base + a.Sx * ly, (A)(vr * w2), (A)(vi * w2));
-            }
-          } else {
-            SpreadCell<A>::add(tile, a.tileElems, loc[n] + lx, (A)vr, (A)vi);
+          for (int ly = 0; ly < W; ++ly) {
+            const R w2 = w[W + ly];
+            SpreadCell<A>::add(tile, a.tileElems, base + a.Sx * ly, (A)(vr * w2), (A)(vi * w2));
           }
+        } else {
+          SpreadCell<A>::add(tile, a.tileElems, (int)pk + lx, (A)vr, (A)vi);
         }
       }
     }
@@ -462,18 +467,13 @@ __device__ __forceinline__ void spread_tile_item(const KArgs& a, int item,
   // padded tile -> this item's scratch slot (dense, px fastest), plain stores
   C* out = scratch + (size_t)item * a.padCells;
   for (int i = threadIdx.x; i < a.padCells; i += NT) {
-    int px = i, py = 0, pz = 0;
+    int px = i, py = 0;
     if (D >= 2) {
       const int q = (int)fdiv((uint32_t)i, a.divP0);
       px = i - q * a.Pd[0];
       py = q;
-      if (D == 3) {
-        const int q2 = (int)fdiv((uint32_t)q, a.divP1);
-        py = q - q2 * a.Pd[1];
-        pz = q2;
-      }
     }
-    const auto v = SpreadCell<A>::get(tile, a.tileElems, px + a.Sx * py + a.Sxy * pz, inv);
+    const auto v = SpreadCell<A>::get(tile, a.tileElems, px + a.Sx * py, 1.0);
     out[i] = C{(R)v.x, (R)v.y};
   }
 }
@@ -568,104 +568,72 @@ __global__ __launch_bounds__(256) void k_fold(KArgs a, typename cpx<R>::T* __res
 // ---------------------------------------------------------------------------
 // Interpolation
 // ---------------------------------------------------------------------------
-// One work item of interpolation: stage the padded tile, gather every node.
+// One work item of interpolation (D <= 2): stage the padded tile, gather
+// every node with G lanes per node.
 template <typename R, int D, int W, int NT>
 __device__ __forceinline__ void interp_tile_item(const KArgs& a, int item,
                                                  const typename cpx<R>::T* __restrict__ grid,
                                                  typename cpx<R>::T* __restrict__ fhat) {
+  static_assert(D <= 2, "3-D interpolation lives in k_tile3.hip");
   using C = typename cpx<R>::T;
   using L = TileLayout<R, D, W>;
   constexpr int NB = Batch<D, R>::NB;
   constexpr int NW = NT / 64;
+  constexpr unsigned LM = NodeRec<R, D, W>::LOCBITS >= 32 ? 0xFFFFFFFFu : (1u << NodeRec<R, D, W>::LOCBITS) - 1u;
   extern __shared__ __align__(16) char smem[];
   const int4 it = a.items[item];
   C* tile = (C*)smem;
-  R* wts = (R*)(smem + L::wts_off(a.tileElems));
-  int* loc = (int*)(smem + L::loc_off(a.tileElems, NB));
-  int* pj = (int*)(smem + L::pj_off(a.tileElems, NB));
+  char* lrec = smem + L::rec_off(a.tileElems);
   int t0[3] = {0, 0, 0};
   tile_origin(a, D, it.x, t0);
 
   // stage the padded tile
   for (int i = threadIdx.x; i < a.padCells; i += NT) {
-    int px = i, py = 0, pz = 0;
+    int px = i, py = 0;
     if (D >= 2) {
       const int q = (int)fdiv((uint32_t)i, a.divP0);
       px = i - q * a.Pd[0];
       py = q;
-      if (D == 3) {
-        const int q2 = (int)fdiv((uint32_t)q, a.divP1);
-        py = q - q2 * a.Pd[1];
-        pz = q2;
-      }
     }
     long long gi = wrapidx((long long)t0[0] - a.m + 1 + px, a.Nt[0]);
     if (D >= 2) gi += wrapidx((long long)t0[1] - a.m + 1 + py, a.Nt[1]) * a.gstr[1];
-    if (D == 3) gi += wrapidx((long long)t0[2] - a.m + 1 + pz, a.Nt[2]) * a.gstr[2];
-    tile[px + a.Sx * py + a.Sxy * pz] = grid[gi];
+    tile[px + a.Sx * py] = grid[gi];
   }
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int G = LanesPerNode<W>::value;
+  constexpr int NPW = 64 / G;
+  const int slot = lane / G, lx = lane & (G - 1);
   for (int b0 = it.y; b0 < it.z; b0 += NB) {
     const int nb = min(NB, it.z - b0);
     __syncthreads();
-    phase_a<R, D, W, NT, false>(a, nullptr, b0, nb, t0, wts, nullptr, loc, pj);
+    phase_a<R, D, W, NT, false>(a, nullptr, b0, nb, lrec, nullptr, (R)1);
     __syncthreads();
-    if constexpr (D == 3) {
-      for (int n = wave; n < nb; n += NW) {
-        const R* w = wts + n * 3 * W;
-        R w3[W];
+    for (int n0 = wave * NPW; n0 < nb; n0 += NW * NPW) {
+      const int n = n0 + slot;
+      R ar = 0, ai = 0;
+      if (n < nb && lx < W) {
+        const R* w = rec_w<R, D, W>(lrec, n);
+        const unsigned pk = rec_loc<R, D, W>(lrec, n);
+        if constexpr (D == 2) {
+          const int base = ((int)(pk & LM) + lx) + a.Sx * (int)(pk >> 16);
 #pragma unroll
-        for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
-        const int bx = loc[n * 3], by = loc[n * 3 + 1], bz = loc[n * 3 + 2];
-        R ar = 0, ai = 0;
-        for (int t = lane; t < W * W; t += 64) {
-          const int ly = t / W, lx = t - ly * W;
-          const int base = (bx + lx) + a.Sx * (by + ly) + a.Sxy * bz;
-          R sr = 0, si = 0;
-#pragma unroll
-          for (int lz = 0; lz < W; ++lz) {
-            const C v = tile[base + a.Sxy * lz];
-            sr = fma(w3[lz], v.x, sr);
-            si = fma(w3[lz], v.y, si);
+          for (int ly = 0; ly < W; ++ly) {
+            const C v = tile[base + a.Sx * ly];
+            ar = fma(w[W + ly], v.x, ar);
+            ai = fma(w[W + ly], v.y, ai);
           }
-          const R wxy = w[lx] * w[W + ly];
-          ar = fma(wxy, sr, ar);
-          ai = fma(wxy, si, ai);
+          ar *= w[lx];
+          ai *= w[lx];
+        } else {
+          const C v = tile[(int)pk + lx];
+          ar = w[lx] * v.x;
+          ai = w[lx] * v.y;
         }
-        ar = group_sum<64>(ar);
-        ai = group_sum<64>(ai);
-        if (lane == 0) fhat[pj[n]] = C{ar * (R)a.oscale, ai * (R)a.oscale};
       }
-    } else {
-      constexpr int G = LanesPerNode<W>::value;
-      constexpr int NPW = 64 / G;
-      const int slot = lane / G, lx = lane & (G - 1);
-      for (int n0 = wave * NPW; n0 < nb; n0 += NW * NPW) {
-        const int n = n0 + slot;
-        R ar = 0, ai = 0;
-        if (n < nb && lx < W) {
-          const R* w = wts + n * D * W;
-          if constexpr (D == 2) {
-            const int base = (loc[n * 2] + lx) + a.Sx * loc[n * 2 + 1];
-#pragma unroll
-            for (int ly = 0; ly < W; ++ly) {
-              const C v = tile[base + a.Sx * ly];
-              ar = fma(w[W + ly], v.x, ar);
-              ai = fma(w[W + ly], v.y, ai);
-            }
-            ar *= w[lx];
-            ai *= w[lx];
-          } else {
-            const C v = tile[loc[n] + lx];
-            ar = w[lx] * v.x;
-            ai = w[lx] * v.y;
-          }
-        }
-        ar = group_sum<G>(ar);
-        ai = group_sum<G>(ai);
-        if (lx == 0 && n < nb) fhat[pj[n]] = C{ar * (R)a.oscale, ai * (R)a.oscale};
-      }
+      ar = group_sum<G>(ar);
+      ai = group_sum<G>(ai);
+      if (lx == 0 && n < nb) fhat[rec_pj<R, D, W>(lrec, n)] = C{ar * (R)a.oscale, ai * (R)a.oscale};
     }
   }
 }

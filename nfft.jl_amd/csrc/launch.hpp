@@ -57,6 +57,9 @@ hipError_t launch_deconv_batch(bool dbl, int nblk, hipStream_t s, const DArgs* d
 hipError_t launch_deconv_t_batch(bool dbl, int nblk, hipStream_t s, const DArgs* da, const int4* work, int nwork,
                                  const void* const* in, void* const* out);
 
+// Node records (weights, local cell, caller index) of every work item.
+hipError_t launch_node_records(int D, bool dbl, int W, const KArgs& a, void* rec, hipStream_t s);
+
 hipError_t launch_deconv(bool dbl, const DArgs& a, hipStream_t s, const void* f, void* g);
 hipError_t launch_deconv_t(bool dbl, const DArgs& a, hipStream_t s, const void* g, void* f);
 hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s);

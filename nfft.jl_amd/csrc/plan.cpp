@@ -314,7 +314,7 @@ struct nfftm_plan_s {
   DevBuf<int> d_perm;
   DevBuf<int4> d_items;
   DevBuf<int> d_tstart;
-  DevBuf<char> d_scratch;
+  DevBuf<char> d_scratch, d_rec;
   int nitems = 0;
   int gen = 0;        // incremented whenever the node arrays are rebuilt
   double wmax = 1.0;  // bound of one (normalised) window value
@@ -414,6 +414,8 @@ struct nfftm_plan_s {
     a.perm = d_perm.p;
     a.items = d_items.p;
     a.nitems = nitems;
+    a.rec = d_rec.p;
+    a.recBytes = node_rec_bytes(D, dbl() ? sizeof(double) : sizeof(float), W);
     a.wmaxD = std::pow(wmax, (double)D);
     a.scratch = d_scratch.p;
     a.tstart = d_tstart.p;
@@ -649,6 +651,10 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
     HIPCHK(hipMemcpy(p.d_tstart.p, tstart.data(), tstart.size() * sizeof(int), hipMemcpyHostToDevice));
     const size_t pad = (size_t)p.Pd[0] * p.Pd[1] * p.Pd[2];
     HIPCHK(p.d_scratch.alloc(std::max<size_t>(1, items.size()) * pad * p.celem()));
+    // node records in tile order, built on the device
+    const size_t rb = (size_t)node_rec_bytes(D, p.dbl() ? sizeof(double) : sizeof(float), p.W);
+    HIPCHK(p.d_rec.alloc(std::max<size_t>(1, (size_t)M) * rb));
+    HIPCHK(launch_node_records(D, p.dbl(), p.W, p.kargs(), p.d_rec.p, p.stream));
   }
   return NFFTM_OK;
 }
