@@ -21,6 +21,7 @@ struct Geo3 {
   static constexpr int SXY = SX * P;
   static constexpr int CELLS = SXY * P;                 // elements of one LDS plane
   static constexpr int PAD = P * P * P;                 // dense padded cells
+  static constexpr int SLOT = (PAD + 15) & ~15;         // scratch slot, 128-byte multiple
   static constexpr size_t TILE_BYTES = 2 * (size_t)CELLS * sizeof(R);
   static constexpr int RB = NodeRec<R, 3, W>::BYTES;
   static constexpr size_t REC_OFF = (TILE_BYTES + 15) & ~(size_t)15;
@@ -34,6 +35,36 @@ struct Geo3 {
 };
 
 __device__ __forceinline__ int wrap1(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
+
+// Block layout of a scratch slot (grids of whole tiles, at least 3 per
+// dimension, m <= T): per dimension the padded range splits into the left
+// halo (m - 1 cells, belonging to the previous tile), the interior (T cells)
+// and the right halo (m cells, belonging to the next tile); the 27 products
+// are stored as contiguous x-fastest blocks, the interior block first, so
+// that the fold reads every block once and in full.
+template <int W, int T>
+struct Blocks3 {
+  static constexpr int m = W / 2;
+  static constexpr __host__ __device__ int ext(int i) { return i == 0 ? m - 1 : i == 1 ? T : m; }
+  static constexpr __host__ __device__ int org(int i) { return i == 0 ? 0 : i == 1 ? m - 1 : m - 1 + T; }
+  static constexpr __host__ __device__ int size(int b) { return ext(b % 3) * ext((b / 3) % 3) * ext(b / 9); }
+  static constexpr __host__ __device__ int off(int b) {
+    if (b == 13) return 0;
+    int o = size(13);
+    for (int q = 0; q < b; ++q)
+      if (q != 13) o += size(q);
+    return o;
+  }
+};
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [I0, N).
+template <int I0, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I0 < N) {
+    f(std::integral_constant<int, I0>{});
+    static_for<I0 + 1, N>(f);
+  }
+}
 
 // Sum over the 64 lanes; the result is valid in lane 63.
 template <typename R>
@@ -221,8 +252,24 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
   }
   __syncthreads();
 
-  // padded tile -> this item's scratch slot, one padded row per 32 lanes
-  C* out = (C*)a.scratch + (size_t)item * G::PAD;
+  // padded tile -> this item's scratch slot
+  C* out = (C*)a.scratch + (size_t)item * G::SLOT;
+  if (a.blocked3) {
+    using B = Blocks3<W, G::T>;
+    static_for<0, 27>([&](auto bc) {
+      constexpr int b = decltype(bc)::value;
+      constexpr int ex = B::ext(b % 3), ey = B::ext((b / 3) % 3), ez = B::ext(b / 9);
+      constexpr int ox = B::org(b % 3), oy = B::org((b / 3) % 3), oz = B::org(b / 9);
+      constexpr int off = B::off(b), n = ex * ey * ez;
+      C* dst = out + off;
+      for (int i = threadIdx.x; i < n; i += NT) {
+        const int lx = i % ex, q = i / ex;
+        const int ly = q % ey, lz = q / ey;
+        dst[i] = SpreadCell<R>::get(tile, G::CELLS, (ox + lx) + G::SX * (oy + ly) + G::SXY * (oz + lz), inv);
+      }
+    });
+    return;
+  }
   const int px = threadIdx.x & 31;
   for (int r = threadIdx.x >> 5; r < G::P * G::P; r += NT / 32) {
     if (px < G::P) {
@@ -306,7 +353,7 @@ __global__ __launch_bounds__(256) void k_fold3(KArgs a, typename cpx<R>::T* __re
         const int t = qx.t + a.ntiles[0] * (qy.t + a.ntiles[1] * qz.t);
         const int i0 = a.tstart[t], i1 = a.tstart[t + 1];
         for (int itm = i0; itm < i1; ++itm) {
-          const C* s = scr + (size_t)itm * G::PAD + (x - qx.off) + P * ((y - qy.off) + P * (-qz.off));
+          const C* s = scr + (size_t)itm * G::SLOT + (x - qx.off) + P * ((y - qy.off) + P * (-qz.off));
 #pragma unroll
           for (int k = 0; k < T; ++k) {
             const int z = z0 + k;
@@ -325,6 +372,60 @@ __global__ __launch_bounds__(256) void k_fold3(KArgs a, typename cpx<R>::T* __re
 #pragma unroll
     for (int k = 0; k < T; ++k)
       if (k < E) grid[((long long)(z0 + k) * Ny + y) * Nx + x] = acc[k];
+  }
+}
+
+// Fold for block-layout slots: output tile o gathers its own tile's interior
+// block and, from each of its 26 neighbours o + d, the halo block facing it
+// (d = -1: the neighbour's right halo over local [0, m); d = +1: its left halo
+// over [T - m + 1, T)), for every work item of those tiles.  Thread (x, y)
+// owns a column of T cells in registers.
+template <typename R, int W>
+__global__ __launch_bounds__(256) void k_fold3b(KArgs a, typename cpx<R>::T* __restrict__ grid) {
+  using C = typename cpx<R>::T;
+  using G = Geo3<R, W>;
+  using B = Blocks3<W, G::T>;
+  constexpr int T = G::T, m = W / 2;
+  const C* scr = (const C*)a.scratch;
+  const int tile = blockIdx.x;
+  const int ntx = a.ntiles[0], nty = a.ntiles[1], ntz = a.ntiles[2];
+  const int ox = tile % ntx, oq = tile / ntx;
+  const int oy = oq % nty, oz = oq / nty;
+  const int lx = threadIdx.x % T, ly = threadIdx.x / T;
+  const bool mine = threadIdx.x < T * T;
+  C acc[T];
+#pragma unroll
+  for (int k = 0; k < T; ++k) acc[k] = C{0, 0};
+  static_for<0, 27>([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    constexpr int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+    constexpr int bx = 1 - dx, by = 1 - dy, bz = 1 - dz;
+    constexpr int sx = dx == 1 ? T - m + 1 : 0, sy = dy == 1 ? T - m + 1 : 0, sz = dz == 1 ? T - m + 1 : 0;
+    constexpr int zlo = dz == 1 ? T - m + 1 : 0, zhi = dz == -1 ? m : T;
+    constexpr int ex = B::ext(bx), ey = B::ext(by);
+    constexpr int off = B::off(bx + 3 * by + 9 * bz);
+    const bool in = mine && (dy == 0 || (dy == -1 ? ly < m : ly >= T - m + 1)) &&
+                    (dx == 0 || (dx == -1 ? lx < m : lx >= T - m + 1));
+    const int tx = (ox + dx + ntx) % ntx, ty = (oy + dy + nty) % nty, tz = (oz + dz + ntz) % ntz;
+    const int t = tx + ntx * (ty + nty * tz);
+    const int i0 = a.tstart[t], i1 = a.tstart[t + 1];
+    for (int itm = i0; itm < i1; ++itm) {
+      const C* s = scr + (size_t)itm * G::SLOT + off + (lx - sx) + ex * (ly - sy);
+#pragma unroll
+      for (int k = zlo; k < zhi; ++k) {
+        if (in) {
+          const C v = s[ex * ey * (k - sz)];
+          acc[k].x += v.x;
+          acc[k].y += v.y;
+        }
+      }
+    }
+  });
+  if (mine) {
+    const int x = ox * T + lx, y = oy * T + ly, z0 = oz * T;
+    const int Nx = a.Nt[0], Ny = a.Nt[1];
+#pragma unroll
+    for (int k = 0; k < T; ++k) grid[((long long)(z0 + k) * Ny + y) * Nx + x] = acc[k];
   }
 }
 
@@ -369,7 +470,10 @@ static hipError_t fold3(const KArgs& a, hipStream_t s, void* grid) {
   using C = typename cpx<R>::T;
   constexpr int T = Geo3<R, W>::T;
   static_assert(T * T <= 256, "fold3 maps one thread per tile column");
-  hipLaunchKernelGGL((k_fold3<R, W>), dim3((unsigned)a.ntile_all), dim3(256), 0, s, a, (C*)grid);
+  if (a.blocked3)
+    hipLaunchKernelGGL((k_fold3b<R, W>), dim3((unsigned)a.ntile_all), dim3(256), 0, s, a, (C*)grid);
+  else
+    hipLaunchKernelGGL((k_fold3<R, W>), dim3((unsigned)a.ntile_all), dim3(256), 0, s, a, (C*)grid);
   return hipGetLastError();
 }
 

@@ -302,6 +302,19 @@ struct nfftm_plan_s {
   double sigma_in = 2.0, sigma = 2.0, b = 0.0, wscale = 1.0;
   i64 N[MAXD] = {0}, Nt[MAXD] = {0}, P = 1, Q = 1, M = 0, LUTSize = 0;
   bool dbl() const { return prec == NFFTM_F64; }
+  // 3-D scratch slots in halo/interior block layout: whole tiles per
+  // dimension and halos (m - 1, m cells) no wider than a tile
+  bool blocked3() const {
+    if (D != 3 || !tiled) return false;
+    for (int d = 0; d < 3; ++d)
+      if (Nt[d] % T[d] != 0 || T[d] < m) return false;
+    return true;
+  }
+  // cells of one scratch slot (3-D: rounded to a 128-byte multiple)
+  size_t slot_cells() const {
+    const size_t pad = (size_t)Pd[0] * Pd[1] * Pd[2];
+    return D == 3 ? (pad + 15) & ~(size_t)15 : pad;
+  }
   size_t celem() const { return dbl() ? 16 : 8; }
   bool tiled = true;
   hipStream_t stream = nullptr;
@@ -415,6 +428,7 @@ struct nfftm_plan_s {
     a.items = d_items.p;
     a.nitems = nitems;
     a.rec = d_rec.p;
+    a.blocked3 = blocked3() ? 1 : 0;
     a.recBytes = node_rec_bytes(D, dbl() ? sizeof(double) : sizeof(float), W);
     a.wmaxD = std::pow(wmax, (double)D);
     a.scratch = d_scratch.p;
@@ -649,8 +663,7 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
   if (p.tiled) {
     HIPCHK(p.d_tstart.alloc(tstart.size()));
     HIPCHK(hipMemcpy(p.d_tstart.p, tstart.data(), tstart.size() * sizeof(int), hipMemcpyHostToDevice));
-    const size_t pad = (size_t)p.Pd[0] * p.Pd[1] * p.Pd[2];
-    HIPCHK(p.d_scratch.alloc(std::max<size_t>(1, items.size()) * pad * p.celem()));
+    HIPCHK(p.d_scratch.alloc(std::max<size_t>(1, items.size()) * p.slot_cells() * p.celem()));
     // node records in tile order, built on the device
     const size_t rb = (size_t)node_rec_bytes(D, p.dbl() ? sizeof(double) : sizeof(float), p.W);
     HIPCHK(p.d_rec.alloc(std::max<size_t>(1, (size_t)M) * rb));
