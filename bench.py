@@ -221,6 +221,8 @@ def main():
     ap.add_argument("--ndft-fwd", type=int, default=1024, help="sampled nodes for the NDFT check")
     ap.add_argument("--ndft-adj", type=int, default=64, help="sampled grid points for the NDFT check")
     ap.add_argument("--workload", default="3d", choices=("3d", "1d", "2d-spiral", "batch"))
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the tile shape of both plans before timing (plan set-up)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -249,6 +251,8 @@ def main():
 
     log("bench: creating FP32 and FP64 plans")
     p32 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float32)
+    if args.tune:
+        log(f"bench: tuned FP32 tiles {p32.tune()}")
     Q = int(np.prod(p32.Ntilde))
     f32 = torch.as_tensor(f_h, device=dev)
     fh32 = torch.as_tensor(fh_h, device=dev)
@@ -287,6 +291,8 @@ def main():
 
     # FP64 plan on the same workload
     p64 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float64)
+    if args.tune:
+        log(f"bench: tuned FP64 tiles {p64.tune()}")
     f64 = f32.to(torch.complex128)
     fh64 = fh32.to(torch.complex128)
     y64 = torch.empty(M, dtype=torch.complex128, device=dev)
@@ -392,6 +398,7 @@ def main():
         "f32_over_f64": t64 / t32,
         "errors": err,
         "stage_ms": stage_ms,
+        "tiles": {"f32": list(p32.tile_shape()), "f64": list(p64.tile_shape())},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

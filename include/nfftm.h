@@ -148,6 +148,13 @@ int nfftm_profile_read(nfftm_plan p, double ms[8], int64_t count[8]);
  * block size is fixed per dimension (src/precomputation.jl:59-77). */
 int nfftm_plan_tune(nfftm_plan p);
 
+/* Set the tile shape explicitly: D <= 2: tile edge (clipped to the grid);
+ * D = 3: geometry variant (0: cubic tile within 160 KiB of LDS, one
+ * workgroup per CU; 1: flat tile within 80 KiB, two per CU); threads 256,
+ * 512 or 1024 (D = 3: 512 or 1024).  NFFTM_ERR_UNSUPPORTED (default shape
+ * restored) when it does not fit. */
+int nfftm_plan_set_tiles(nfftm_plan p, int tile, int threads);
+
 /* Current tile edge per dimension (0 beyond D, or when the plan uses the
  * node-parallel kernels) and workgroup size of the gridding kernels. */
 int nfftm_tile_shape(nfftm_plan p, int tile[3], int* threads);

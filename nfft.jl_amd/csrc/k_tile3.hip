@@ -9,29 +9,53 @@
 
 namespace nfftm {
 
-// Tile edge per precision and stencil width W = 2m, the largest that keeps
-// the padded tile (T + W - 1)^3 plus one batch of node records within
-// 160 KiB of LDS.
-template <typename R, int W>
+// 3-D tile geometry per precision, stencil width W = 2m and variant V: tile
+// T x T x TZ, padded P x P x PZ (P = T + W - 1).  V = 0: cubic, the largest
+// edge whose padded tile plus a batch of node records fits 160 KiB of LDS
+// (one workgroup per CU).  V = 1: the largest T x T x TZ (T, TZ in 16, 8, 4)
+// within 80 KiB, so that two workgroups share a CU and one's staging and
+// flush overlap the other's taps, at the price of more halo per tile.
+template <typename R, int W, int V>
 struct Geo3 {
-  static constexpr int T = sizeof(R) == 4 ? (W <= 10 ? 16 : W <= 12 ? 12 : W <= 14 ? 10 : 8)
-                                          : (W <= 8 ? 12 : W <= 12 ? 8 : W <= 14 ? 6 : 4);
+  static constexpr int RB = NodeRec<R, 3, W>::BYTES;
+  static constexpr __host__ __device__ int pad_of(int t) { return t + W - 1; }
+  static constexpr __host__ __device__ int sx_of(int t) { return pad_of(t) % 16 == 7 ? pad_of(t) + 1 : pad_of(t); }
+  static constexpr __host__ __device__ size_t tile_bytes(int t, int tz) {
+    return ((2 * (size_t)sx_of(t) * pad_of(t) * pad_of(tz) * sizeof(R)) + 15) & ~(size_t)15;
+  }
+  static constexpr __host__ __device__ size_t batch_bytes(int nb) { return (size_t)nb * RB + (size_t)nb * 2 * sizeof(R); }
+  static constexpr size_t CAP = V == 0 ? 163840 : 81920;
+  // V = 1 choice, encoded t * 32 + tz
+  static constexpr __host__ __device__ int flat_tile() {
+    int best = 4 * 32 + 4, vol = 0;
+    const int ts[3] = {16, 8, 4};
+    for (int i = 0; i < 3; ++i)
+      for (int j = i; j < 3; ++j) {
+        const int t = ts[i], tz = ts[j];
+        if (tile_bytes(t, tz) + batch_bytes(64) <= CAP && t * t * tz > vol) {
+          vol = t * t * tz;
+          best = t * 32 + tz;
+        }
+      }
+    return best;
+  }
+  static constexpr int TC = sizeof(R) == 4 ? (W <= 10 ? 16 : W <= 12 ? 12 : W <= 14 ? 10 : 8)
+                                           : (W <= 8 ? 12 : W <= 12 ? 8 : W <= 14 ? 6 : 4);
+  static constexpr int T = V == 0 ? TC : flat_tile() / 32;
+  static constexpr int TZ = V == 0 ? TC : flat_tile() % 32;
   static constexpr int P = T + W - 1;
-  static constexpr int SX = (P % 16 == 7) ? P + 1 : P;  // W = 8: 8x8 lanes on distinct banks
+  static constexpr int PZ = TZ + W - 1;
+  static constexpr int SX = sx_of(T);  // W = 8: 8x8 lanes on distinct banks
   static constexpr int SXY = SX * P;
-  static constexpr int CELLS = SXY * P;                 // elements of one LDS plane
-  static constexpr int PAD = P * P * P;                 // dense padded cells
+  static constexpr int CELLS = SXY * PZ;                // elements of one LDS plane
+  static constexpr int PAD = P * P * PZ;                // dense padded cells
   static constexpr int SLOT = (PAD + 15) & ~15;         // scratch slot, 128-byte multiple
   static constexpr size_t TILE_BYTES = 2 * (size_t)CELLS * sizeof(R);
-  static constexpr int RB = NodeRec<R, 3, W>::BYTES;
   static constexpr size_t REC_OFF = (TILE_BYTES + 15) & ~(size_t)15;
-  static constexpr size_t batch_bytes(int nb) { return (size_t)nb * RB + (size_t)nb * 2 * sizeof(R); }
-  static constexpr int NB = REC_OFF + batch_bytes(256) <= 163840   ? 256
-                            : REC_OFF + batch_bytes(128) <= 163840 ? 128
-                                                                   : 64;
+  static constexpr int NB = REC_OFF + batch_bytes(256) <= CAP ? 256 : REC_OFF + batch_bytes(128) <= CAP ? 128 : 64;
   static constexpr size_t FV_OFF = REC_OFF + (size_t)NB * RB;
   static constexpr size_t LDS = FV_OFF + (size_t)NB * 2 * sizeof(R);
-  static_assert(LDS <= 163840, "3-D tile does not fit LDS");
+  static constexpr bool VALID = LDS <= CAP;
 };
 
 __device__ __forceinline__ int wrap1(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
@@ -42,12 +66,12 @@ __device__ __forceinline__ int wrap1(int x, int n) { return x < 0 ? x + n : (x >
 // and the right halo (m cells, belonging to the next tile); the 27 products
 // are stored as contiguous x-fastest blocks, the interior block first, so
 // that the fold reads every block once and in full.
-template <int W, int T>
+template <int W, int T, int TZ>
 struct Blocks3 {
   static constexpr int m = W / 2;
-  static constexpr __host__ __device__ int ext(int i) { return i == 0 ? m - 1 : i == 1 ? T : m; }
-  static constexpr __host__ __device__ int org(int i) { return i == 0 ? 0 : i == 1 ? m - 1 : m - 1 + T; }
-  static constexpr __host__ __device__ int size(int b) { return ext(b % 3) * ext((b / 3) % 3) * ext(b / 9); }
+  static constexpr __host__ __device__ int ext(int i, int t = T) { return i == 0 ? m - 1 : i == 1 ? t : m; }
+  static constexpr __host__ __device__ int org(int i, int t = T) { return i == 0 ? 0 : i == 1 ? m - 1 : m - 1 + t; }
+  static constexpr __host__ __device__ int size(int b) { return ext(b % 3) * ext((b / 3) % 3) * ext(b / 9, TZ); }
   static constexpr __host__ __device__ int off(int b) {
     if (b == 13) return 0;
     int o = size(13);
@@ -82,11 +106,11 @@ __device__ __forceinline__ R wave_total(R v) {
 // is copied to LDS; 16 lanes per node, 4 nodes per wavefront, each lane
 // walking ceil(W^2 / 16) (x, y) columns of the footprint along z, and a
 // 16-lane DPP reduction per node.
-template <typename R, int W, int NT>
+template <typename R, int W, int NT, int V>
 __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
                                                 typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
-  using G = Geo3<R, W>;
+  using G = Geo3<R, W, V>;
   constexpr int NW = NT / 64;
   constexpr int NCOL = (W * W + 15) / 16;
   extern __shared__ __align__(16) char smem[];
@@ -100,7 +124,7 @@ __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::
     const int tx = it.x % a.ntiles[0], q = it.x / a.ntiles[0];
     t0[0] = tx * G::T;
     t0[1] = (q % a.ntiles[1]) * G::T;
-    t0[2] = (q / a.ntiles[1]) * G::T;
+    t0[2] = (q / a.ntiles[1]) * G::TZ;
   }
   const int Nx = a.Nt[0], Ny = a.Nt[1], Nz = a.Nt[2];
   const long long sy = a.gstr[1], sz = a.gstr[2];
@@ -109,7 +133,7 @@ __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::
   {
     const int px = threadIdx.x & 31;
     const int gx = wrap1(t0[0] - a.m + 1 + px, Nx);
-    for (int r = threadIdx.x >> 5; r < G::P * G::P; r += NT / 32) {
+    for (int r = threadIdx.x >> 5; r < G::P * G::PZ; r += NT / 32) {
       const int py = r % G::P, pz = r / G::P;
       if (px < G::P) {
         const long long g = (long long)wrap1(t0[2] - a.m + 1 + pz, Nz) * sz +
@@ -168,10 +192,10 @@ __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::
 // one LDS atomic per tap (FP32: packed 32+32-bit fixed point with a per-item
 // scale from a strict bound; FP64: two FP64 planes); the padded tile goes to
 // the item's scratch slot.
-template <typename R, int W, int NT>
+template <typename R, int W, int NT, int V>
 __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
-  using G = Geo3<R, W>;
+  using G = Geo3<R, W, V>;
   constexpr int NW = NT / 64;
   extern __shared__ __align__(16) char smem[];
   const int item = xcd_item(a.nitems);
@@ -255,11 +279,11 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
   // padded tile -> this item's scratch slot
   C* out = (C*)a.scratch + (size_t)item * G::SLOT;
   if (a.blocked3) {
-    using B = Blocks3<W, G::T>;
+    using B = Blocks3<W, G::T, G::TZ>;
     static_for<0, 27>([&](auto bc) {
       constexpr int b = decltype(bc)::value;
-      constexpr int ex = B::ext(b % 3), ey = B::ext((b / 3) % 3), ez = B::ext(b / 9);
-      constexpr int ox = B::org(b % 3), oy = B::org((b / 3) % 3), oz = B::org(b / 9);
+      constexpr int ex = B::ext(b % 3), ey = B::ext((b / 3) % 3), ez = B::ext(b / 9, G::TZ);
+      constexpr int ox = B::org(b % 3), oy = B::org((b / 3) % 3), oz = B::org(b / 9, G::TZ);
       constexpr int off = B::off(b), n = ex * ey * ez;
       C* dst = out + off;
       for (int i = threadIdx.x; i < n; i += NT) {
@@ -271,7 +295,7 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
     return;
   }
   const int px = threadIdx.x & 31;
-  for (int r = threadIdx.x >> 5; r < G::P * G::P; r += NT / 32) {
+  for (int r = threadIdx.x >> 5; r < G::P * G::PZ; r += NT / 32) {
     if (px < G::P) {
       const int li = px + G::SX * (r % G::P) + G::SXY * (r / G::P);
       out[px + G::P * r] = SpreadCell<R>::get(tile, G::CELLS, li, inv);
@@ -316,11 +340,11 @@ __device__ __forceinline__ int pieces(int o, int Nt, int nt, int m1, Piece* pc) 
   return n;
 }
 
-template <typename R, int W>
+template <typename R, int W, int V>
 __global__ __launch_bounds__(256) void k_fold3(KArgs a, typename cpx<R>::T* __restrict__ grid) {
   using C = typename cpx<R>::T;
-  using G = Geo3<R, W>;
-  constexpr int T = G::T, P = G::P;
+  using G = Geo3<R, W, V>;
+  constexpr int T = G::T, P = G::P, TZ = G::TZ, PZ = G::PZ;
   const C* scr = (const C*)a.scratch;
   const int m1 = a.m - 1;
   const int tile = blockIdx.x;
@@ -332,16 +356,16 @@ __global__ __launch_bounds__(256) void k_fold3(KArgs a, typename cpx<R>::T* __re
   if (threadIdx.x == 0) {
     np[0] = pieces<T, P>(ox, Nx, a.ntiles[0], m1, px);
     np[1] = pieces<T, P>(oy, Ny, a.ntiles[1], m1, py);
-    np[2] = pieces<T, P>(oz, Nz, a.ntiles[2], m1, pz);
+    np[2] = pieces<TZ, PZ>(oz, Nz, a.ntiles[2], m1, pz);
   }
   __syncthreads();
   const int npx = np[0], npy = np[1], npz = np[2];
   const int lx = threadIdx.x % T, ly = threadIdx.x / T;
-  const int x = ox * T + lx, y = oy * T + ly, z0 = oz * T;
+  const int x = ox * T + lx, y = oy * T + ly, z0 = oz * TZ;
   const bool mine = threadIdx.x < T * T && x < Nx && y < Ny;
-  C acc[T];
+  C acc[TZ];
 #pragma unroll
-  for (int k = 0; k < T; ++k) acc[k] = C{0, 0};
+  for (int k = 0; k < TZ; ++k) acc[k] = C{0, 0};
   for (int iz = 0; iz < npz; ++iz) {
     const Piece qz = pz[iz];
     for (int iy = 0; iy < npy; ++iy) {
@@ -355,7 +379,7 @@ __global__ __launch_bounds__(256) void k_fold3(KArgs a, typename cpx<R>::T* __re
         for (int itm = i0; itm < i1; ++itm) {
           const C* s = scr + (size_t)itm * G::SLOT + (x - qx.off) + P * ((y - qy.off) + P * (-qz.off));
 #pragma unroll
-          for (int k = 0; k < T; ++k) {
+          for (int k = 0; k < TZ; ++k) {
             const int z = z0 + k;
             if (z >= qz.lo && z < qz.hi && in) {
               const C v = s[P * P * z];
@@ -368,9 +392,9 @@ __global__ __launch_bounds__(256) void k_fold3(KArgs a, typename cpx<R>::T* __re
     }
   }
   if (mine) {
-    const int E = min(T, Nz - z0);
+    const int E = min(TZ, Nz - z0);
 #pragma unroll
-    for (int k = 0; k < T; ++k)
+    for (int k = 0; k < TZ; ++k)
       if (k < E) grid[((long long)(z0 + k) * Ny + y) * Nx + x] = acc[k];
   }
 }
@@ -380,12 +404,12 @@ __global__ __launch_bounds__(256) void k_fold3(KArgs a, typename cpx<R>::T* __re
 // (d = -1: the neighbour's right halo over local [0, m); d = +1: its left halo
 // over [T - m + 1, T)), for every work item of those tiles.  Thread (x, y)
 // owns a column of T cells in registers.
-template <typename R, int W>
+template <typename R, int W, int V>
 __global__ __launch_bounds__(256) void k_fold3b(KArgs a, typename cpx<R>::T* __restrict__ grid) {
   using C = typename cpx<R>::T;
-  using G = Geo3<R, W>;
-  using B = Blocks3<W, G::T>;
-  constexpr int T = G::T, m = W / 2;
+  using G = Geo3<R, W, V>;
+  using B = Blocks3<W, G::T, G::TZ>;
+  constexpr int T = G::T, TZ = G::TZ, m = W / 2;
   const C* scr = (const C*)a.scratch;
   const int tile = blockIdx.x;
   const int ntx = a.ntiles[0], nty = a.ntiles[1], ntz = a.ntiles[2];
@@ -393,15 +417,15 @@ __global__ __launch_bounds__(256) void k_fold3b(KArgs a, typename cpx<R>::T* __r
   const int oy = oq % nty, oz = oq / nty;
   const int lx = threadIdx.x % T, ly = threadIdx.x / T;
   const bool mine = threadIdx.x < T * T;
-  C acc[T];
+  C acc[TZ];
 #pragma unroll
-  for (int k = 0; k < T; ++k) acc[k] = C{0, 0};
+  for (int k = 0; k < TZ; ++k) acc[k] = C{0, 0};
   static_for<0, 27>([&](auto dc) {
     constexpr int d = decltype(dc)::value;
     constexpr int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
     constexpr int bx = 1 - dx, by = 1 - dy, bz = 1 - dz;
-    constexpr int sx = dx == 1 ? T - m + 1 : 0, sy = dy == 1 ? T - m + 1 : 0, sz = dz == 1 ? T - m + 1 : 0;
-    constexpr int zlo = dz == 1 ? T - m + 1 : 0, zhi = dz == -1 ? m : T;
+    constexpr int sx = dx == 1 ? T - m + 1 : 0, sy = dy == 1 ? T - m + 1 : 0, sz = dz == 1 ? TZ - m + 1 : 0;
+    constexpr int zlo = dz == 1 ? TZ - m + 1 : 0, zhi = dz == -1 ? m : TZ;
     constexpr int ex = B::ext(bx), ey = B::ext(by);
     constexpr int off = B::off(bx + 3 * by + 9 * bz);
     const bool in = mine && (dy == 0 || (dy == -1 ? ly < m : ly >= T - m + 1)) &&
@@ -422,65 +446,79 @@ __global__ __launch_bounds__(256) void k_fold3b(KArgs a, typename cpx<R>::T* __r
     }
   });
   if (mine) {
-    const int x = ox * T + lx, y = oy * T + ly, z0 = oz * T;
+    const int x = ox * T + lx, y = oy * T + ly, z0 = oz * TZ;
     const int Nx = a.Nt[0], Ny = a.Nt[1];
 #pragma unroll
-    for (int k = 0; k < T; ++k) grid[((long long)(z0 + k) * Ny + y) * Nx + x] = acc[k];
+    for (int k = 0; k < TZ; ++k) grid[((long long)(z0 + k) * Ny + y) * Nx + x] = acc[k];
+  }
+}
+
+template <typename R, int W, int V>
+static hipError_t go3(bool spread, int NT, const KArgs& a, hipStream_t s, const void* in, void* out) {
+  using C = typename cpx<R>::T;
+  using G = Geo3<R, W, V>;
+  if constexpr (!G::VALID) {
+    return hipErrorInvalidConfiguration;
+  } else {
+    const int nblk = 8 * ((a.nitems + 7) / 8);
+    const int lds = (int)G::LDS;
+    if (spread) {
+      auto k = NT == 1024 ? k_spread3<R, W, 1024, V> : k_spread3<R, W, 512, V>;
+      if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
+        return hipErrorInvalidConfiguration;
+      hipLaunchKernelGGL(k, dim3(nblk), dim3(NT), lds, s, a, (const C*)in);
+      (void)out;
+    } else {
+      auto k = NT == 1024 ? k_interp3<R, W, 1024, V> : k_interp3<R, W, 512, V>;
+      if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
+        return hipErrorInvalidConfiguration;
+      hipLaunchKernelGGL(k, dim3(nblk), dim3(NT), lds, s, a, (const C*)in, (C*)out);
+    }
+    return hipGetLastError();
   }
 }
 
 template <typename R, int W>
-static hipError_t go3(bool spread, int NT, const KArgs& a, hipStream_t s, const void* in, void* out) {
-  using C = typename cpx<R>::T;
-  using G = Geo3<R, W>;
-  const int nblk = 8 * ((a.nitems + 7) / 8);
-  const int lds = (int)G::LDS;
-  if (spread) {
-    auto k = NT == 1024 ? k_spread3<R, W, 1024> : k_spread3<R, W, 512>;
-    if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
-      return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(k, dim3(nblk), dim3(NT), lds, s, a, (const C*)in);
-    (void)out;
-  } else {
-    auto k = NT == 1024 ? k_interp3<R, W, 1024> : k_interp3<R, W, 512>;
-    if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds) != hipSuccess)
-      return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(k, dim3(nblk), dim3(NT), lds, s, a, (const C*)in, (C*)out);
-  }
-  return hipGetLastError();
+static hipError_t go3v(bool spread, int NT, const KArgs& a, hipStream_t s, const void* in, void* out) {
+  return a.tvar ? go3<R, W, 1>(spread, NT, a, s, in, out) : go3<R, W, 0>(spread, NT, a, s, in, out);
 }
 
 template <typename R>
 static hipError_t go3w(bool spread, int W, int NT, const KArgs& a, hipStream_t s, const void* in, void* out) {
   switch (W) {
-    case 2: return go3<R, 2>(spread, NT, a, s, in, out);
-    case 4: return go3<R, 4>(spread, NT, a, s, in, out);
-    case 6: return go3<R, 6>(spread, NT, a, s, in, out);
-    case 8: return go3<R, 8>(spread, NT, a, s, in, out);
-    case 10: return go3<R, 10>(spread, NT, a, s, in, out);
-    case 12: return go3<R, 12>(spread, NT, a, s, in, out);
-    case 14: return go3<R, 14>(spread, NT, a, s, in, out);
-    case 16: return go3<R, 16>(spread, NT, a, s, in, out);
+    case 2: return go3v<R, 2>(spread, NT, a, s, in, out);
+    case 4: return go3v<R, 4>(spread, NT, a, s, in, out);
+    case 6: return go3v<R, 6>(spread, NT, a, s, in, out);
+    case 8: return go3v<R, 8>(spread, NT, a, s, in, out);
+    case 10: return go3v<R, 10>(spread, NT, a, s, in, out);
+    case 12: return go3v<R, 12>(spread, NT, a, s, in, out);
+    case 14: return go3v<R, 14>(spread, NT, a, s, in, out);
+    case 16: return go3v<R, 16>(spread, NT, a, s, in, out);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <typename R, int W>
+template <typename R, int W, int V>
 static hipError_t fold3(const KArgs& a, hipStream_t s, void* grid) {
   using C = typename cpx<R>::T;
-  constexpr int T = Geo3<R, W>::T;
-  static_assert(T * T <= 256, "fold3 maps one thread per tile column");
-  if (a.blocked3)
-    hipLaunchKernelGGL((k_fold3b<R, W>), dim3((unsigned)a.ntile_all), dim3(256), 0, s, a, (C*)grid);
-  else
-    hipLaunchKernelGGL((k_fold3<R, W>), dim3((unsigned)a.ntile_all), dim3(256), 0, s, a, (C*)grid);
-  return hipGetLastError();
+  using G = Geo3<R, W, V>;
+  if constexpr (!G::VALID) {
+    return hipErrorInvalidConfiguration;
+  } else {
+    static_assert(G::T * G::T <= 256, "fold3 maps one thread per tile column");
+    if (a.blocked3)
+      hipLaunchKernelGGL((k_fold3b<R, W, V>), dim3((unsigned)a.ntile_all), dim3(256), 0, s, a, (C*)grid);
+    else
+      hipLaunchKernelGGL((k_fold3<R, W, V>), dim3((unsigned)a.ntile_all), dim3(256), 0, s, a, (C*)grid);
+    return hipGetLastError();
+  }
 }
 
 hipError_t launch_fold_d3(bool dbl, int W, const KArgs& a, hipStream_t s, void* grid) {
-#define NFFTM_F3(WW) \
-  case WW:           \
-    return dbl ? fold3<double, WW>(a, s, grid) : fold3<float, WW>(a, s, grid);
+#define NFFTM_F3(WW)                                                                          \
+  case WW:                                                                                    \
+    if (a.tvar) return dbl ? fold3<double, WW, 1>(a, s, grid) : fold3<float, WW, 1>(a, s, grid); \
+    return dbl ? fold3<double, WW, 0>(a, s, grid) : fold3<float, WW, 0>(a, s, grid);
   switch (W) {
     NFFTM_F3(2) NFFTM_F3(4) NFFTM_F3(6) NFFTM_F3(8) NFFTM_F3(10) NFFTM_F3(12) NFFTM_F3(14) NFFTM_F3(16)
     default:
@@ -496,25 +534,36 @@ hipError_t launch_tile_d3(bool spread, bool dbl, int W, int NT, const KArgs& a, 
   return dbl ? go3w<double>(spread, W, NT, a, s, in, out) : go3w<float>(spread, W, NT, a, s, in, out);
 }
 
-template <typename R>
-static void geo3(int W, int* T, size_t* lds, int* P) {
-#define NFFTM_G(WW)               \
-  case WW:                        \
-    *T = Geo3<R, WW>::T;          \
-    *P = Geo3<R, WW>::P;          \
-    *lds = Geo3<R, WW>::LDS;      \
+template <typename R, int V>
+static void geo3(int W, int* T, size_t* lds, int* P, bool* valid) {
+#define NFFTM_G(WW)                    \
+  case WW:                             \
+    T[0] = Geo3<R, WW, V>::T;          \
+    T[1] = Geo3<R, WW, V>::TZ;         \
+    P[0] = Geo3<R, WW, V>::P;          \
+    P[1] = Geo3<R, WW, V>::PZ;         \
+    *lds = Geo3<R, WW, V>::LDS;        \
+    *valid = Geo3<R, WW, V>::VALID;    \
     break;
   switch (W) {
     NFFTM_G(2) NFFTM_G(4) NFFTM_G(6) NFFTM_G(8) NFFTM_G(10) NFFTM_G(12) NFFTM_G(14) NFFTM_G(16)
     default:
-      *T = 0;
+      T[0] = T[1] = 0;
+      P[0] = P[1] = 0;
+      *lds = 0;
+      *valid = false;
   }
 #undef NFFTM_G
 }
 
-void tile3_geometry(bool dbl, int W, int* T, size_t* lds, int* P) {
-  if (dbl) geo3<double>(W, T, lds, P);
-  else geo3<float>(W, T, lds, P);
+void tile3_geometry(bool dbl, int W, int V, int* T, size_t* lds, int* P, bool* valid) {
+  if (dbl) {
+    if (V) geo3<double, 1>(W, T, lds, P, valid);
+    else geo3<double, 0>(W, T, lds, P, valid);
+  } else {
+    if (V) geo3<float, 1>(W, T, lds, P, valid);
+    else geo3<float, 0>(W, T, lds, P, valid);
+  }
 }
 
 }  // namespace nfftm

@@ -106,6 +106,7 @@ struct KArgs {
   const char* rec;    // node records in sorted order (NodeRec layout), tiled plans
   int recBytes;
   int blocked3;       // 3-D scratch slots stored as 27 halo/interior blocks
+  int tvar;           // 3-D tile geometry variant (compile-time tile edge)
   // spreading
   double wmaxD;        // bound of any product of D window values
   void* scratch;       // nitems padded tiles (dense, px fastest)

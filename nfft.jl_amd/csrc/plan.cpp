@@ -334,6 +334,7 @@ struct nfftm_plan_s {
 
   // tiling
   int T[3] = {1, 1, 1}, ntiles[3] = {1, 1, 1}, Pd[3] = {1, 1, 1}, Sx = 1, Sxy = 1, tileElems = 1, NT = 256;
+  int tvar = 0;  // 3-D tile geometry variant
   size_t lds = 0;
 
   std::unique_ptr<FFTStage> fft;
@@ -429,6 +430,7 @@ struct nfftm_plan_s {
     a.nitems = nitems;
     a.rec = d_rec.p;
     a.blocked3 = blocked3() ? 1 : 0;
+    a.tvar = tvar;
     a.recBytes = node_rec_bytes(D, dbl() ? sizeof(double) : sizeof(float), W);
     a.wmaxD = std::pow(wmax, (double)D);
     a.scratch = d_scratch.p;
@@ -479,12 +481,10 @@ int max_cover(i64 T, i64 P, i64 Nt, int m) {
     }
     return c;
   };
-  if (nt > 4096) {  // only the wrap neighbourhood can exceed the interior bound
-    for (i64 x = 0; x < std::min(Nt, 2 * P); ++x) best = std::max(best, count(x));
-    for (i64 x = std::max<i64>(0, Nt - 2 * P); x < Nt; ++x) best = std::max(best, count(x));
-  } else {
-    for (i64 x = 0; x < Nt; ++x) best = std::max(best, count(x));
-  }
+  // away from the periodic wrap every point has the interior count above;
+  // only points within 2P of either end can exceed it
+  for (i64 x = 0; x < std::min(Nt, 2 * P); ++x) best = std::max(best, count(x));
+  for (i64 x = std::max<i64>(0, Nt - 2 * P); x < Nt; ++x) best = std::max(best, count(x));
   return best;
 }
 
@@ -522,23 +522,43 @@ bool set_tile_shape(nfftm_plan_s& p, int t) {
 // until the tile fits LDS); D = 1 tiles of 4096 (2048 in FP64) likewise.
 // Grids smaller than a padded tile fail tiles_ok and take the node-parallel
 // path.
+bool tiles_ok(const nfftm_plan_s& p);
+
+// 3-D geometry of variant V (compile-time tile of k_tile3.hip); false if
+// that variant has no tile fitting its LDS budget.
+bool set_tile3(nfftm_plan_s& p, int V) {
+  int T[2] = {0, 0}, P[2] = {0, 0};
+  size_t lds = 0;
+  bool valid = false;
+  tile3_geometry(p.dbl(), p.W, V, T, &lds, P, &valid);
+  if (!valid) return false;
+  p.tvar = V;
+  for (int d = 0; d < 3; ++d) {
+    p.T[d] = T[d == 2 ? 1 : 0];
+    p.Pd[d] = P[d == 2 ? 1 : 0];
+    p.ntiles[d] = (int)((p.Nt[d] + p.T[d] - 1) / p.T[d]);
+  }
+  p.Sx = P[0];
+  p.Sxy = P[0] * P[0];
+  p.tileElems = P[0] * P[0] * P[1];
+  p.lds = lds;
+  return true;
+}
+
 void choose_tiles(nfftm_plan_s& p) {
   const int D = p.D;
   const bool dbl = p.dbl();
   if (D == 3) {
+    // the flat two-per-CU geometry unless it costs more than 1.7x the halo
+    // of the cubic one (measured on the 256^3 grid: faster at m = 4 in both
+    // precisions)
     p.NT = 1024;
-    int T = 0, P = 0;
-    size_t lds = 0;
-    tile3_geometry(dbl, p.W, &T, &lds, &P);
-    for (int d = 0; d < 3; ++d) {
-      p.T[d] = T;
-      p.Pd[d] = P;
-      p.ntiles[d] = (int)((p.Nt[d] + T - 1) / T);
-    }
-    p.Sx = P;
-    p.Sxy = P * P;
-    p.tileElems = P * P * P;
-    p.lds = lds;
+    set_tile3(p, 1);
+    const double h1 = (double)p.Pd[0] * p.Pd[1] * p.Pd[2] / ((double)p.T[0] * p.T[1] * p.T[2]);
+    set_tile3(p, 0);
+    const double h0 = (double)p.Pd[0] * p.Pd[1] * p.Pd[2] / ((double)p.T[0] * p.T[1] * p.T[2]);
+    if (h1 <= 1.7 * h0 && set_tile3(p, 1) && tiles_ok(p)) return;
+    set_tile3(p, 0);
     return;
   }
   p.NT = 256;
@@ -855,17 +875,29 @@ int time_gridding(nfftm_plan_s& p, void* fh, double* ms) {
 }
 
 // Candidate shapes: D = 2 tile edges 16..128, D = 1 edges 512..8192, with 256
-// or 512 threads (maxNT caps it); D = 3 keeps its compile-time tile and picks
-// 512 or 1024 threads.  Leaves the fastest applied.
+// or 512 threads (maxNT caps it); D = 3: both compile-time geometries with 512
+// or 1024 threads.  Plans of more than 2^20 nodes are timed on every k-th node
+// (the same spatial distribution at a bounded binning cost); the nodes are
+// binned once per geometry.  Leaves the fastest applied to all nodes.
 int tune_plan(nfftm_plan_s& p, int maxNT, std::vector<TuneChoice>* log) {
   if (!p.tiled || p.M == 0) return NFFTM_OK;
+  const i64 Mfull = p.M;
+  const std::vector<double> kfull = p.k;
+  const i64 stride = (Mfull + (1 << 20) - 1) >> 20;
+  std::vector<double> ksub;
+  if (stride > 1) {
+    for (i64 j = 0; j < Mfull; j += stride) ksub.insert(ksub.end(), &kfull[(size_t)j * p.D], &kfull[(size_t)(j + 1) * p.D]);
+  }
+  const double* ktune = stride > 1 ? ksub.data() : kfull.data();
+  const i64 Mtune = stride > 1 ? (i64)(ksub.size() / p.D) : Mfull;
   DevBuf<char> fh;
-  HIPCHK(fh.alloc((size_t)p.M * p.celem()));
-  HIPCHK(launch_fill0(fh.p, (size_t)p.M * p.celem(), p.stream));
-  std::vector<std::pair<int, int>> cand;
+  HIPCHK(fh.alloc((size_t)Mtune * p.celem()));
+  HIPCHK(launch_fill0(fh.p, (size_t)Mtune * p.celem(), p.stream));
+  std::vector<std::pair<int, int>> cand;  // (D <= 2: tile edge, D = 3: variant), threads
   if (p.D == 3) {
-    for (int nt : {512, 1024})
-      if (nt <= maxNT) cand.push_back({0, nt});
+    for (int v : {0, 1})
+      for (int nt : {512, 1024})
+        if (nt <= maxNT) cand.push_back({v, nt});
   } else {
     const std::vector<int> ts = p.D == 2 ? std::vector<int>{16, 32, 64, 128}
                                          : std::vector<int>{512, 1024, 2048, 4096, 8192};
@@ -875,10 +907,13 @@ int tune_plan(nfftm_plan_s& p, int maxNT, std::vector<TuneChoice>* log) {
   }
   TuneChoice best;
   best.ms = 1e30;
+  int binned = -1;
   for (auto& c : cand) {
-    if (p.D != 3) {
-      if (!set_tile_shape(p, c.first) || !tiles_ok(p)) continue;
-      if (int rc = set_nodes(p, p.k.data(), p.M)) return rc;
+    const bool ok = p.D == 3 ? set_tile3(p, c.first) : set_tile_shape(p, c.first);
+    if (!ok || !tiles_ok(p)) continue;
+    if (c.first != binned) {
+      if (int rc = set_nodes(p, ktune, Mtune)) return rc;
+      binned = c.first;
     }
     p.NT = c.second;
     double ms = 0;
@@ -888,14 +923,12 @@ int tune_plan(nfftm_plan_s& p, int maxNT, std::vector<TuneChoice>* log) {
   }
   if (best.NT == 0) {  // nothing fitted: restore the default shape
     choose_tiles(p);
-    return set_nodes(p, p.k.data(), p.M);
+    return set_nodes(p, kfull.data(), Mfull);
   }
-  if (p.D != 3) {
-    set_tile_shape(p, best.t);
-    if (int rc = set_nodes(p, p.k.data(), p.M)) return rc;
-  }
+  if (p.D == 3) set_tile3(p, best.t);
+  else set_tile_shape(p, best.t);
   p.NT = best.NT;
-  return NFFTM_OK;
+  return set_nodes(p, kfull.data(), Mfull);
 }
 
 }  // namespace
@@ -1415,6 +1448,23 @@ int nfftm_plan_tune(nfftm_plan p) {
   g_err.clear();
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
   return tune_plan(*p, 1024, nullptr);
+}
+
+int nfftm_plan_set_tiles(nfftm_plan p, int tile, int threads) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if (!p->tiled) return fail(NFFTM_ERR_UNSUPPORTED, "the plan uses the node-parallel kernels");
+  const bool ok_nt = p->D == 3 ? (threads == 512 || threads == 1024) : (threads == 256 || threads == 512 || threads == 1024);
+  if (!ok_nt) return fail(NFFTM_ERR_UNSUPPORTED, "threads = %d not available for D = %d", threads, p->D);
+  HIPCHK(hipStreamSynchronize(p->stream));
+  const bool ok = p->D == 3 ? (tile == 0 || tile == 1) && set_tile3(*p, tile) : tile > 0 && set_tile_shape(*p, tile);
+  if (!ok || !tiles_ok(*p)) {
+    choose_tiles(*p);
+    if (int rc = set_nodes(*p, p->k.data(), p->M)) return rc;
+    return fail(NFFTM_ERR_UNSUPPORTED, "tile %d does not fit this plan", tile);
+  }
+  p->NT = threads;
+  return set_nodes(*p, p->k.data(), p->M);
 }
 
 int nfftm_tile_shape(nfftm_plan p, int* tile, int* threads) {

@@ -437,6 +437,13 @@ class NFFTPlan:
         _check(self._lib.nfftm_plan_tune(self._handle))
         return self.tile_shape()
 
+    def set_tiles(self, tile, threads):
+        """Set the tile shape (nfftm_plan_set_tiles): tile edge for D <= 2,
+        geometry variant 0 / 1 for D = 3."""
+        torch.cuda.current_stream().synchronize()
+        _check(self._lib.nfftm_plan_set_tiles(self._handle, int(tile), int(threads)))
+        return self.tile_shape()
+
     def tile_shape(self):
         """(tile edge per dimension, workgroup size) of the gridding kernels."""
         t = (ctypes.c_int * 3)()

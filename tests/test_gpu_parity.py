@@ -266,3 +266,26 @@ def test_3d_whole_tile_grids(N, m):
     tol = 1e-5 if m >= 4 else 1e-3  # m = 2 is a coarse window; FP32 only adds rounding
     assert rel(p32.H * fh, x_ref) < tol
     assert rel(p32 * f, y_ref) < tol
+
+
+@pytest.mark.parametrize("variant,threads", [(0, 512), (0, 1024), (1, 512), (1, 1024)])
+@pytest.mark.parametrize("N", [(32, 32, 32), (20, 24, 36)], ids=lambda n: "x".join(map(str, n)))
+def test_3d_tile_variants(variant, threads, N):
+    """Both 3-D tile geometries (cubic, and flat within half the LDS) and
+    both workgroup sizes, on a whole-tile grid (block fold) and an irregular
+    one (piece fold), against the oracle."""
+    banner(f"3-D variant={variant} threads={threads} N={N}")
+    rng = np.random.default_rng([47, variant, threads, N[0]])
+    k = rng.random((3, 4000)) - 0.5
+    fh, f = crandn(rng, 4000), crandn(rng, N)
+    orc = O.OraclePlan(k, N, 4, 2.0, O.LINEAR, True)
+    x_ref, y_ref = orc.adjoint(fh), orc.forward(f)
+    for dt, tol in ((torch.float64, F64_VS_ORACLE), (torch.float32, 1e-5)):
+        p = nfftm.plan_nfft(k, N, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=dt)
+        try:
+            shape = p.set_tiles(variant, threads)
+        except nfftm.NFFTError:
+            continue  # this variant does not fit the grid
+        assert shape[1] == threads
+        assert rel(p.H * fh, x_ref) < tol
+        assert rel(p * f, y_ref) < tol
