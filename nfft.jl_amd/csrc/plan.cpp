@@ -517,9 +517,7 @@ bool set_tile_shape(nfftm_plan_s& p, int t) {
   return bytes <= budget;
 }
 
-// Default tile shape: D = 3 uses the compile-time geometry of k_tile3.hip;
-// D = 2 tiles of 64^2 (halved while the grid has fewer than 256 tiles, and
-// until the tile fits LDS); D = 1 tiles of 4096 (2048 in FP64) likewise.
+// Default tile shape (D = 3: the compile-time geometries of k_tile3.hip).
 // Grids smaller than a padded tile fail tiles_ok and take the node-parallel
 // path.
 bool tiles_ok(const nfftm_plan_s& p);
@@ -561,13 +559,15 @@ void choose_tiles(nfftm_plan_s& p) {
     set_tile3(p, 0);
     return;
   }
-  p.NT = 256;
+  // D = 2: 16^2 tiles; D = 1: 2048 cells (FP64: 1024); 512 threads -- the
+  // autotuner's choice on the 2-D spiral, the batch of uneven 2-D plans and
+  // the edge-clustered 1-D workload (nfftm_plan_tune explores the rest)
+  p.NT = 512;
   if (D == 2) {
-    int t = 64;
-    while (t > 16 && (double)p.Nt[0] * p.Nt[1] / ((double)t * t) < 256) t /= 2;
+    int t = 16;
     while (!set_tile_shape(p, t) && t > 4) t /= 2;
   } else {
-    int t = dbl ? 2048 : 4096;
+    int t = dbl ? 1024 : 2048;
     while (t > 256 && (double)p.Nt[0] / t < 256) t /= 2;
     while (!set_tile_shape(p, t) && t > 64) t /= 2;
   }
