@@ -42,6 +42,13 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by a 256 MiB elementwise copy in the same run: factors 2.0 / 1.0);
+# profiles/nfft3d_128cubed_M2p21_hbm_traffic.md holds the passes and the
+# command.  Keyed by (workload, stage) for the default FP32 plan of that
+# workload (3-D: k_spread3<float, 8, 1024, 1>, flat 16 x 16 x 8 tiles).
+MEASURED_TRAFFIC = {("3d", "spreading"): 1115303727, ("3d", "interpolation"): 563884712}
+
 STAGES = ("deconvolve", "fft_forward", "interpolation", "grid_clear", "spreading", "fft_backward",
           "deconvolve_transpose", "node_gradient")
 
@@ -351,9 +358,11 @@ def main():
             "node_gradient")
     dom = max((s for s in mine if stage_ms[s]), key=lambda s: stage_ms[s])
     achieved = sb[dom] / (stage_ms[dom] * 1e-3)
+    traffic = None if args.tune else MEASURED_TRAFFIC.get((args.workload, dom))
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                "bytes_per_launch": sb[dom], "avg_ms": stage_ms[dom]}
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "bytes_per_launch": sb[dom], "avg_ms": stage_ms[dom],
+                "traffic_over_algorithmic": None if traffic is None else traffic / sb[dom]}
     B_total = (P * sc + 4 * Q * sc + M * (D * 8 + sc)) + (2 * P * sc + 3 * Q * sc + M * (D * 8 + sc))
 
     cpu = None
