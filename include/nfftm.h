@@ -58,7 +58,9 @@ enum { NFFTM_KAISER_BESSEL = 0 };
 /* Plan flags. */
 enum {
   NFFTM_FLAG_BLOCKING = 1, /* tiled kernels (NFFTParams.blocking, default true) */
-  NFFTM_FLAG_TIMING = 2    /* record per-stage device times (TimingStats) */
+  NFFTM_FLAG_TIMING = 2,   /* record per-stage device times (TimingStats) */
+  NFFTM_FLAG_GRAPHS = 4    /* replay repeated (input, output) transforms from HIP
+                              graphs captured at the pair's second call */
 };
 
 /* Plan creation: replaces NFFTPlan(k, N; m, sigma, window, precompute,

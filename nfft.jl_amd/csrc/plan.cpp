@@ -349,7 +349,28 @@ struct nfftm_plan_s {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<double> k;  // caller's nodes (D x M), kept for copy()
 
+  // Replayed transforms: the launch sequence of one direction for one
+  // (input, output) pair, captured into a HIP graph at the pair's second call
+  // (first calls run directly), dropped when the node arrays are rebuilt.
+  struct GraphEntry {
+    int dir;
+    const void* in;
+    void* out;
+    int uses;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  hipStream_t cap = nullptr;  // capture stream
+  bool graphs_ok = true;      // false after a failed capture: direct launches only
+  void drop_graphs() {
+    for (auto& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
+
   ~nfftm_plan_s() {
+    drop_graphs();
+    if (cap) (void)hipStreamDestroy(cap);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& pr : prof_ev) {
@@ -619,6 +640,7 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
   if (int rc = check_nodes(k, (i64)D * M)) return rc;
   p.M = M;
   p.gen++;
+  p.drop_graphs();
   if (k != p.k.data()) p.k.assign(k, k + (size_t)D * M);
   std::vector<double> ks((size_t)D * M);
   std::vector<int> perm;
@@ -1272,10 +1294,74 @@ int nfftm_nodes(nfftm_plan p, const double* k_host, int64_t M) {
   return set_nodes(*p, k_host, M);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Launch sequence of one direction on p.stream: forward = deconvolve, FFT,
+// interpolate; adjoint = spread (+ fold), backward FFT, deconvolve_transpose.
+int enqueue_transform(nfftm_plan_s& p, bool fwd, const void* in, void* out) {
+  if (fwd) {
+    if (int rc = run_deconv(p, true, in, p.d_grid.p)) return rc;
+    if (int rc = p.fft->exec(true, p.d_grid.p, p.celem(), p.stream)) return rc;
+    if (p.M) return run_conv(p, false, p.d_grid.p, out);
+    return NFFTM_OK;
+  }
+  if (int rc = run_conv(p, true, in, p.d_grid.p)) return rc;
+  if (int rc = p.fft->exec(false, p.d_grid.p, p.celem(), p.stream)) return rc;
+  return run_deconv(p, false, p.d_grid.p, out);
+}
+
+// Graph replay of enqueue_transform for a repeated (input, output) pair.
+int graph_transform(nfftm_plan_s& p, bool fwd, const void* in, void* out) {
+  const int dir = fwd ? 0 : 1;
+  for (auto& g : p.graphs) {
+    if (g.dir != dir || g.in != in || g.out != out) continue;
+    ++g.uses;
+    if (!g.exec && p.graphs_ok) {
+      if (!p.cap) HIPCHK(hipStreamCreate(&p.cap));
+      hipStream_t saved = p.stream;
+      p.stream = p.cap;
+      int rc = NFFTM_ERR_HIP;
+      hipGraph_t graph = nullptr;
+      if (hipStreamBeginCapture(p.cap, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        rc = enqueue_transform(p, fwd, in, out);
+        if (hipStreamEndCapture(p.cap, &graph) != hipSuccess) rc = NFFTM_ERR_HIP;
+      }
+      p.stream = saved;
+      if (rc == NFFTM_OK && graph && hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) != hipSuccess)
+        g.exec = nullptr;
+      if (graph) (void)hipGraphDestroy(graph);
+      if (!g.exec) {  // capture not possible here: launch directly from now on
+        p.graphs_ok = false;
+        (void)hipGetLastError();
+        g_err.clear();
+      }
+    }
+    if (g.exec) {
+      HIPCHK(hipGraphLaunch(g.exec, p.stream));
+      return NFFTM_OK;
+    }
+    return enqueue_transform(p, fwd, in, out);
+  }
+  if (p.graphs.size() >= 8) {
+    if (p.graphs.front().exec) (void)hipGraphExecDestroy(p.graphs.front().exec);
+    p.graphs.erase(p.graphs.begin());
+  }
+  p.graphs.push_back(nfftm_plan_s::GraphEntry{dir, in, out, 1, nullptr});
+  return enqueue_transform(p, fwd, in, out);
+}
+
+}  // namespace
+
+extern "C" {
+
 int nfftm_forward(nfftm_plan p, const void* f, void* fhat) {
   g_err.clear();
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
   if ((!f && p->P) || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
+  if ((p->flags & NFFTM_FLAG_GRAPHS) && !p->profiling && !(p->flags & NFFTM_FLAG_TIMING))
+    return graph_transform(*p, true, f, fhat);
   Timer t(*p);
   if (int rc = t.mark(0)) return rc;
   if (int rc = run_deconv(*p, true, f, p->d_grid.p)) return rc;
@@ -1296,6 +1382,8 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
   g_err.clear();
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
   if ((!f && p->P) || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
+  if ((p->flags & NFFTM_FLAG_GRAPHS) && !p->profiling && !(p->flags & NFFTM_FLAG_TIMING))
+    return graph_transform(*p, false, fhat, f);
   Timer t(*p);
   if (int rc = t.mark(0)) return rc;
   if (int rc = run_conv(*p, true, fhat, p->d_grid.p)) return rc;

@@ -289,3 +289,33 @@ def test_3d_tile_variants(variant, threads, N):
         assert shape[1] == threads
         assert rel(p.H * fh, x_ref) < tol
         assert rel(p * f, y_ref) < tol
+
+
+@pytest.mark.parametrize("N", [(40, 36), (24, 20, 16)], ids=lambda n: "x".join(map(str, n)))
+def test_graph_replay_single_plan(N):
+    """A (input, output) pair seen twice is replayed from a captured graph:
+    every call -- fresh data in the same buffers, unrelated device work in
+    between, a nodes! in the middle -- matches the oracle."""
+    banner(f"graph replay N={N}")
+    rng = np.random.default_rng([53, len(N)])
+    D, M = len(N), 3000
+    k = rng.random((D, M)) - 0.5
+    p = nfftm.plan_nfft(k, N, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=torch.float64, graphs=True)
+    f = torch.empty(N, dtype=torch.complex128, device="cuda")
+    fh = torch.empty(M, dtype=torch.complex128, device="cuda")
+    y = torch.empty(M, dtype=torch.complex128, device="cuda")
+    x = torch.empty(N, dtype=torch.complex128, device="cuda")
+    for call in range(5):
+        if call == 3:
+            k = rng.random((D, M)) - 0.5
+            p.nodes_(k)
+        fn, fhn = crandn(rng, N), crandn(rng, M)
+        f.copy_(torch.as_tensor(fn))
+        fh.copy_(torch.as_tensor(fhn))
+        p.forward(f, y)
+        p.adjoint(fh, x)
+        _ = torch.randn(1 << 18, device="cuda").sum()
+        torch.cuda.synchronize()
+        orc = O.OraclePlan(k, N, 4, 2.0, O.LINEAR, True)
+        assert rel(y, orc.forward(fn)) < F64_VS_ORACLE, call
+        assert rel(x, orc.adjoint(fhn)) < F64_VS_ORACLE, call
