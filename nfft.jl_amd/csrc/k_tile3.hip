@@ -50,12 +50,17 @@ struct Geo3 {
   static constexpr int CELLS = SXY * PZ;                // elements of one LDS plane
   static constexpr int PAD = P * P * PZ;                // dense padded cells
   static constexpr int SLOT = (PAD + 15) & ~15;         // scratch slot, 128-byte multiple
-  static constexpr size_t TILE_BYTES = 2 * (size_t)CELLS * sizeof(R);
-  static constexpr size_t REC_OFF = (TILE_BYTES + 15) & ~(size_t)15;
-  static constexpr int NB = REC_OFF + batch_bytes(256) <= CAP ? 256 : REC_OFF + batch_bytes(128) <= CAP ? 128 : 64;
-  static constexpr size_t FV_OFF = REC_OFF + (size_t)NB * RB;
-  static constexpr size_t LDS = FV_OFF + (size_t)NB * 2 * sizeof(R);
+  static constexpr size_t TILE_BYTES = (2 * (size_t)CELLS * sizeof(R) + 15) & ~(size_t)15;
+  // LDS: node records of one batch, their values (spreading), then the tile,
+  // so that record fields sit at small immediate offsets
+  static constexpr int NB = TILE_BYTES + batch_bytes(256) <= CAP ? 256 : TILE_BYTES + batch_bytes(128) <= CAP ? 128 : 64;
+  static constexpr size_t REC_OFF = 0;
+  static constexpr size_t FV_OFF = (size_t)NB * RB;
+  static constexpr size_t TILE_OFF = FV_OFF + (size_t)NB * 2 * sizeof(R);
+  static constexpr size_t LDS = TILE_OFF + TILE_BYTES;
   static constexpr bool VALID = LDS <= CAP;
+  static_assert(TILE_OFF % 16 == 0, "tile offset");
+
 };
 
 __device__ __forceinline__ int wrap1(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
@@ -102,12 +107,83 @@ __device__ __forceinline__ R wave_total(R v) {
   return v;
 }
 
+// Rows of the padded tile each thread loads before writing them to LDS
+// (measured on the 256^3 grid: FP32 gains up to 11 rows in flight, FP64 loses
+// beyond 2).
+template <typename R>
+constexpr int STAGE_RB = sizeof(R) == 4 ? 11 : 2;
+
+// Copy the padded tile at grid cell (x0, y0, z0) (periodic) into LDS, one
+// padded row per 32 lanes (P <= 32): the x offsets are per lane, the row
+// offsets per half-wave; each thread issues the loads of RB rows before
+// writing them.  32-bit offsets unless the grid holds 2^32 or more elements.
+template <typename G, int NT, int RB, typename C>
+__device__ __forceinline__ void stage3(const C* __restrict__ grid, C* tile, int x0, int y0, int z0, int Nx, int Ny,
+                                       int Nz, long long sy, long long sz, bool off32) {
+  constexpr int P = G::P, ROWS = G::P * G::PZ, RPP = NT / 32;
+  static_assert(P <= 32, "one padded row per 32 lanes");
+  const int px = threadIdx.x & 31, r0 = threadIdx.x >> 5;
+  const bool lx = px < P;
+  const int gx = wrap1(x0 + (lx ? px : 0), Nx);
+  for (int q0 = 0; q0 < ROWS; q0 += RB * RPP) {
+    C v[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const int r = q0 + b * RPP + r0;
+      if (lx && r < ROWS) {
+        const int py = r % P, pz = r / P;
+        const int gy = wrap1(y0 + py, Ny), gz = wrap1(z0 + pz, Nz);
+        const size_t g = off32 ? (size_t)((unsigned)gz * (unsigned)sz + (unsigned)gy * (unsigned)sy + (unsigned)gx)
+                               : (size_t)(gz * sz + gy * sy + gx);
+        v[b] = grid[g];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const int r = q0 + b * RPP + r0;
+      if (lx && r < ROWS) tile[px + G::SX * (r % P) + G::SXY * (r / P)] = v[b];
+    }
+  }
+}
+
 // Interpolation: the padded tile is staged once; each batch of node records
 // is copied to LDS; 16 lanes per node, 4 nodes per wavefront, each lane
 // walking ceil(W^2 / 16) (x, y) columns of the footprint along z, and a
 // 16-lane DPP reduction per node.
+// The records of one batch of nodes held in registers between their global
+// load and their LDS store, so that the load of the next batch overlaps the
+// taps of this one.
+template <typename G, int NT>
+struct RecStage {
+  static constexpr int N16 = G::NB * G::RB / 16;
+  static constexpr int PER = (N16 + NT - 1) / NT;
+  static_assert(G::RB % 16 == 0, "records are 16-byte blocks");
+  uint4 v[PER];
+  __device__ __forceinline__ void load(const KArgs& a, int b0, int nb) {
+    const uint4* src = (const uint4*)(a.rec + (size_t)b0 * G::RB);
+    const int n16 = nb * (G::RB / 16);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = (int)threadIdx.x + u * NT;
+      if (i < n16) v[u] = src[i];
+    }
+  }
+  __device__ __forceinline__ void store(char* lrec, int nb) const {
+    uint4* dst = (uint4*)lrec;
+    const int n16 = nb * (G::RB / 16);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = (int)threadIdx.x + u * NT;
+      if (i < n16) dst[i] = v[u];
+    }
+  }
+};
+
+// V = 1 geometries hold two workgroups per CU: 2 NT / 256 waves per SIMD
+// (1024 threads: 8 waves, at most 64 VGPRs) where that needs no spills.
 template <typename R, int W, int NT, int V>
-__global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(V == 1 && W <= 12 ? NT / 128 : 1)))
+void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
                                                 typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
   using G = Geo3<R, W, V>;
@@ -117,7 +193,7 @@ __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::
   const int item = xcd_item(a.nitems);
   if (item >= a.nitems) return;
   const int4 it = a.items[item];
-  C* tile = (C*)smem;
+  C* tile = (C*)(smem + G::TILE_OFF);
   char* lrec = smem + G::REC_OFF;
   int t0[3];
   {
@@ -129,19 +205,9 @@ __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::
   const int Nx = a.Nt[0], Ny = a.Nt[1], Nz = a.Nt[2];
   const long long sy = a.gstr[1], sz = a.gstr[2];
 
-  // stage the padded tile, one padded row per 32 lanes
-  {
-    const int px = threadIdx.x & 31;
-    const int gx = wrap1(t0[0] - a.m + 1 + px, Nx);
-    for (int r = threadIdx.x >> 5; r < G::P * G::PZ; r += NT / 32) {
-      const int py = r % G::P, pz = r / G::P;
-      if (px < G::P) {
-        const long long g = (long long)wrap1(t0[2] - a.m + 1 + pz, Nz) * sz +
-                            (long long)wrap1(t0[1] - a.m + 1 + py, Ny) * sy + gx;
-        tile[px + G::SX * py + G::SXY * pz] = grid[g];
-      }
-    }
-  }
+  // stage the padded tile
+  stage3<G, NT, STAGE_RB<R>>(grid, tile, t0[0] - a.m + 1, t0[1] - a.m + 1, t0[2] - a.m + 1, Nx, Ny, Nz, sy, sz,
+                             (unsigned long long)a.gstr[2] * (unsigned long long)Nz <= 0xFFFFFFFFull);
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int quarter = lane >> 4, r16 = lane & 15;
@@ -159,9 +225,7 @@ __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::
       R w3[W];
 #pragma unroll
       for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
-      const unsigned pk = rec_loc<R, 3, W>(lrec, nn);
-      const int bx = pk & 1023, by = (pk >> 10) & 1023, bz = pk >> 20;
-      const C* base0 = tile + bx + G::SX * by + G::SXY * bz;
+      const C* base0 = tile + rec_loc<R, 3, W>(lrec, nn);
       R ar = 0, ai = 0;
 #pragma unroll
       for (int c = 0; c < NCOL; ++c) {
@@ -193,7 +257,8 @@ __global__ __launch_bounds__(NT) void k_interp3(KArgs a, const typename cpx<R>::
 // scale from a strict bound; FP64: two FP64 planes); the padded tile goes to
 // the item's scratch slot.
 template <typename R, int W, int NT, int V>
-__global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(V == 1 && W <= 12 ? NT / 128 : 1)))
+void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
   using G = Geo3<R, W, V>;
   constexpr int NW = NT / 64;
@@ -201,19 +266,30 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
   const int item = xcd_item(a.nitems);
   if (item >= a.nitems) return;
   const int4 it = a.items[item];
-  char* tile = smem;
+  char* tile = smem + G::TILE_OFF;
   char* lrec = smem + G::REC_OFF;
   C* fv = (C*)(smem + G::FV_OFF);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+
+  // the first batch's records; the item's node values, held in registers
+  // (thread t holds nodes it.y + t + h NT)
+  RecStage<G, NT> rs;
+  rs.load(a, it.y, min(G::NB, it.z - it.y));
+  constexpr int HPT = ITEM3 / NT;
+  static_assert(ITEM3 % NT == 0, "item size");
+  C held[HPT];
+#pragma unroll
+  for (int h = 0; h < HPT; ++h) {
+    const int j = it.y + (int)threadIdx.x + h * NT;
+    held[h] = j < it.z ? fhat[a.perm[j]] : C{0, 0};
+  }
 
   // FP32: fixed-point scale s = 2^(30 - e), B = sum_j |f_j|_inf * wmaxD < 2^e
   R fscale = 1, inv = 1;
   if (sizeof(R) == 4) {
     R part = 0;
-    for (int j = it.y + threadIdx.x; j < it.z; j += NT) {
-      const C v = fhat[a.perm[j]];
-      part += fmax(fabs(v.x), fabs(v.y));
-    }
+#pragma unroll
+    for (int h = 0; h < HPT; ++h) part += fmax(fabs(held[h].x), fabs(held[h].y));
     part = wave_total(part);
     R* red = (R*)lrec;
     if (lane == 63) red[wave] = part;
@@ -226,21 +302,27 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
     fscale = (R)ldexp(1.0, 30 - e);
     inv = (R)ldexp(1.0, e - 30);
   }
-  for (int i = threadIdx.x; i < 2 * G::CELLS; i += NT) ((R*)tile)[i] = 0;
+  for (int i = threadIdx.x; i < (int)(G::TILE_BYTES / 16); i += NT) ((uint4*)tile)[i] = make_uint4(0, 0, 0, 0);
 
+  const R sc = fscale * (R)a.oscale;
   for (int b0 = it.y; b0 < it.z; b0 += G::NB) {
     const int nb = min(G::NB, it.z - b0);
     __syncthreads();
-    phase_a<R, 3, W, NT, true>(a, fhat, b0, nb, lrec, fv, fscale * (R)a.oscale);
+    rs.store(lrec, nb);
+#pragma unroll
+    for (int h = 0; h < HPT; ++h) {
+      const int n = it.y + (int)threadIdx.x + h * NT - b0;
+      if (n >= 0 && n < nb) fv[n] = C{held[h].x * sc, held[h].y * sc};
+    }
     __syncthreads();
+    if (b0 + G::NB < it.z) rs.load(a, b0 + G::NB, min(G::NB, it.z - b0 - G::NB));
     for (int n = wave; n < nb; n += NW) {
       const C f = fv[n];
       const R* w = rec_w<R, 3, W>(lrec, n);
       R w3[W];
 #pragma unroll
       for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
-      const unsigned pk = rec_loc<R, 3, W>(lrec, n);
-      const int bx = pk & 1023, by = (pk >> 10) & 1023, bz = pk >> 20;
+      const int cell = (int)rec_loc<R, 3, W>(lrec, n);
 #pragma unroll
       for (int t0l = 0; t0l < W * W; t0l += 64) {
         const int t = t0l + lane;
@@ -248,7 +330,7 @@ __global__ __launch_bounds__(NT) void k_spread3(KArgs a, const typename cpx<R>::
           const int ly = t / W, lx = t - ly * W;
           const R wxy = w[lx] * w[W + ly];
           const R vr = f.x * wxy, vi = f.y * wxy;
-          const int base = (bx + lx) + G::SX * (by + ly) + G::SXY * bz;
+          const int base = cell + lx + G::SX * ly;
           if constexpr (sizeof(R) == 4) {
             unsigned long long* tl = (unsigned long long*)tile + base;
 #pragma unroll
@@ -535,13 +617,14 @@ hipError_t launch_tile_d3(bool spread, bool dbl, int W, int NT, const KArgs& a, 
 }
 
 template <typename R, int V>
-static void geo3(int W, int* T, size_t* lds, int* P, bool* valid) {
+static void geo3(int W, int* T, size_t* lds, int* P, int* SX, bool* valid) {
 #define NFFTM_G(WW)                    \
   case WW:                             \
     T[0] = Geo3<R, WW, V>::T;          \
     T[1] = Geo3<R, WW, V>::TZ;         \
     P[0] = Geo3<R, WW, V>::P;          \
     P[1] = Geo3<R, WW, V>::PZ;         \
+    *SX = Geo3<R, WW, V>::SX;          \
     *lds = Geo3<R, WW, V>::LDS;        \
     *valid = Geo3<R, WW, V>::VALID;    \
     break;
@@ -550,19 +633,20 @@ static void geo3(int W, int* T, size_t* lds, int* P, bool* valid) {
     default:
       T[0] = T[1] = 0;
       P[0] = P[1] = 0;
+      *SX = 0;
       *lds = 0;
       *valid = false;
   }
 #undef NFFTM_G
 }
 
-void tile3_geometry(bool dbl, int W, int V, int* T, size_t* lds, int* P, bool* valid) {
+void tile3_geometry(bool dbl, int W, int V, int* T, size_t* lds, int* P, int* SX, bool* valid) {
   if (dbl) {
-    if (V) geo3<double, 1>(W, T, lds, P, valid);
-    else geo3<double, 0>(W, T, lds, P, valid);
+    if (V) geo3<double, 1>(W, T, lds, P, SX, valid);
+    else geo3<double, 0>(W, T, lds, P, SX, valid);
   } else {
-    if (V) geo3<float, 1>(W, T, lds, P, valid);
-    else geo3<float, 0>(W, T, lds, P, valid);
+    if (V) geo3<float, 1>(W, T, lds, P, SX, valid);
+    else geo3<float, 0>(W, T, lds, P, SX, valid);
   }
 }
 

@@ -259,8 +259,10 @@ __device__ __forceinline__ void gbl_add(R* p, R v) {
 // ---------------------------------------------------------------------------
 // Node record, one per node in tile order, built once per node set: the
 // window weights of every dimension (D x W values of R), the node's cell
-// relative to its tile origin packed in 32 / D bits per dimension, and the
-// caller's index of the node.  16-byte aligned for block copies.
+// relative to its tile origin (D <= 2: packed in 32 / D bits per dimension;
+// D = 3: the LDS element index x + Sx y + Sxy z of that cell in the padded
+// tile), and the caller's index of the node.  16-byte aligned for block
+// copies.
 template <typename R, int D, int W>
 struct NodeRec {
   static constexpr int WBYTES = D * W * (int)sizeof(R);
@@ -292,6 +294,9 @@ __host__ inline size_t tile_layout_bytes(int D, size_t sizeofR, int W, int tileE
   const size_t o = (2 * (size_t)tileElems * acc + 15) & ~(size_t)15;
   return o + (size_t)NB * node_rec_bytes(D, sizeofR, W) + (size_t)NB * 2 * sizeofR;
 }
+
+// Nodes of one 3-D work item at most (items are chunks of one tile's nodes).
+constexpr int ITEM3 = 1024;
 
 // Nodes per phase-A batch.
 template <int D, typename R>
@@ -334,7 +339,8 @@ __global__ __launch_bounds__(256) void k_node_records(KArgs a, char* __restrict_
       const int cw = node_weights<R, W>(a, d, a.ks[(size_t)d * a.M + j], wd);
 #pragma unroll
       for (int l = 0; l < W; ++l) w[d * W + l] = wd[l];
-      pk |= (unsigned)(cw - t0[d]) << (N::LOCBITS * d);
+      if (D == 3) pk += (unsigned)(cw - t0[d]) * (unsigned)(d == 0 ? 1 : d == 1 ? a.Sx : a.Sxy);
+      else pk |= (unsigned)(cw - t0[d]) << (N::LOCBITS * d);
     }
     ((int*)(out + N::WBYTES))[0] = (int)pk;
     ((int*)(out + N::WBYTES))[1] = a.perm[j];

@@ -73,6 +73,6 @@ hipError_t launch_fold_d3(bool dbl, int W, const KArgs& a, hipStream_t s, void* 
 // V = 0: largest cubic tile within 160 KiB of LDS; V = 1: flat tile within
 // 80 KiB (two workgroups per CU).  T[0] / P[0]: x and y tile / padded edge,
 // T[1] / P[1]: z.  valid is false when no tile of that variant fits.
-void tile3_geometry(bool dbl, int W, int V, int* T, size_t* lds, int* P, bool* valid);
+void tile3_geometry(bool dbl, int W, int V, int* T, size_t* lds, int* P, int* SX, bool* valid);
 
 }  // namespace nfftm

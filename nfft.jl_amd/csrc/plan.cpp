@@ -546,10 +546,10 @@ bool tiles_ok(const nfftm_plan_s& p);
 // 3-D geometry of variant V (compile-time tile of k_tile3.hip); false if
 // that variant has no tile fitting its LDS budget.
 bool set_tile3(nfftm_plan_s& p, int V) {
-  int T[2] = {0, 0}, P[2] = {0, 0};
+  int T[2] = {0, 0}, P[2] = {0, 0}, SX = 0;
   size_t lds = 0;
   bool valid = false;
-  tile3_geometry(p.dbl(), p.W, V, T, &lds, P, &valid);
+  tile3_geometry(p.dbl(), p.W, V, T, &lds, P, &SX, &valid);
   if (!valid) return false;
   p.tvar = V;
   for (int d = 0; d < 3; ++d) {
@@ -557,8 +557,8 @@ bool set_tile3(nfftm_plan_s& p, int V) {
     p.Pd[d] = P[d == 2 ? 1 : 0];
     p.ntiles[d] = (int)((p.Nt[d] + p.T[d] - 1) / p.T[d]);
   }
-  p.Sx = P[0];
-  p.Sxy = P[0] * P[0];
+  p.Sx = SX;  // LDS strides of the tile: the 3-D node records hold cell = x + Sx y + Sxy z
+  p.Sxy = SX * P[0];
   p.tileElems = P[0] * P[0] * P[1];
   p.lds = lds;
   return true;
@@ -682,7 +682,7 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
     for (i64 s = 0; s < M; ++s)
       for (int d = 0; d < D; ++d) ks[(size_t)d * M + s] = shifted(perm[s], d);
     // work items: chunks of at most CH nodes of one tile
-    const i64 CH = D == 3 ? 1024 : D == 2 ? 2048 : 4096;
+    const i64 CH = D == 3 ? ITEM3 : D == 2 ? 2048 : 4096;
     tstart.assign(ntile + 1, 0);
     for (i64 t = 0; t < ntile; ++t) {
       tstart[t] = (int)items.size();
