@@ -59,8 +59,18 @@ enum { NFFTM_KAISER_BESSEL = 0 };
 enum {
   NFFTM_FLAG_BLOCKING = 1, /* tiled kernels (NFFTParams.blocking, default true) */
   NFFTM_FLAG_TIMING = 2,   /* record per-stage device times (TimingStats) */
-  NFFTM_FLAG_GRAPHS = 4    /* replay repeated (input, output) transforms from HIP
+  NFFTM_FLAG_GRAPHS = 4,   /* replay repeated (input, output) transforms from HIP
                               graphs captured at the pair's second call */
+  NFFTM_FLAG_ROCFFT = 8    /* FFT stage as rocFFT over the whole oversampled grid
+                              with separate deconvolution passes.  Default on
+                              power-of-two grids (FP32: 16..2048 points per
+                              dimension, FP64: 16..1024, D <= 3): pruned line
+                              FFTs that skip the zero padding (forward) and
+                              the discarded positions (adjoint), with the
+                              deconvolution fused.  Results agree to rounding;
+                              the grid (nfftm_grid) then holds the full FFT
+                              after a forward transform and a partial one
+                              after an adjoint. */
 };
 
 /* Plan creation: replaces NFFTPlan(k, N; m, sigma, window, precompute,

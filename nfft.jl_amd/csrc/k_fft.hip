@@ -1,0 +1,437 @@
+// k_fft.hip -- the FFT stage of a transform on power-of-two grids as pruned
+// line passes, with the deconvolution fused into the dimension-1 pass.
+//
+// The forward transform only fills the image corners of the oversampled
+// grid (N_d of the Nt_d positions per dimension, src/deconvolution.jl:22-45)
+// and the adjoint only keeps those positions of the inverse FFT
+// (src/deconvolution.jl:69-92).  A D-dimensional FFT is D passes of 1-D
+// FFTs along the lines of one dimension, so each pass can skip the lines
+// that are zero (forward) or discarded (adjoint):
+//   forward  X: lines (y, z) in S_y x S_z, input read from f (deconvolution
+//               fused), zero elsewhere;
+//            Y: lines (x, z), z in S_z, input nonzero on S_y only;
+//            Z: all lines (x, y), input nonzero on S_z only;
+//   adjoint  Z: all lines, output kept on S_z;  Y: lines (x, z), z in S_z,
+//               output kept on S_y;  X: lines (y, z) in S_y x S_z, output
+//               kept on S_x and written to f with the transposed
+//               deconvolution fused.
+// S_d is the set of grid positions of the image indices: image index i < Na
+// goes to i - Na + Nt, the rest to i - Na.  Values equal the full FFT of the
+// zero-padded grid (FFTW's forward / unnormalised backward transform,
+// src/implementation.jl:87-88) up to rounding.
+//
+// One workgroup transforms E / L lines of length L = 2^k in LDS (FP32:
+// E = 4096 points, 256 threads, 16 <= L <= 2048; FP64: E = 2048, 128
+// threads, 16 <= L <= 1024): a first radix-2^(k mod 4) pass, then radix-16
+// Stockham passes, L / 16 threads per line, one radix-16 butterfly (or
+// 16 / R radix-R ones) per thread and pass, twiddles from a table of
+// exp(-2 pi i j / L) in LDS.  Strided passes (Y, Z) take E / L lines that
+// are consecutive along x, so that every row of the load and store is a
+// contiguous run of E / L elements.
+#include "launch.hpp"
+
+namespace nfftm {
+
+namespace {
+
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) {
+  return C{a.x + b.x, a.y + b.y};
+}
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) {
+  return C{a.x - b.x, a.y - b.y};
+}
+
+// x * conj(t) (forward, t = (cos, sin) of +2 pi j / L) or x * t (inverse).
+template <bool INV, typename C>
+__device__ __forceinline__ C tmul(C x, C t) {
+  if (INV) return C{x.x * t.x - x.y * t.y, x.y * t.x + x.x * t.y};
+  return C{x.x * t.x + x.y * t.y, x.y * t.x - x.x * t.y};
+}
+
+// cos and sin of 2 pi k / 16, k < 8
+__device__ constexpr double kC16[8] = {1.0,
+                                       0.92387953251128675613,
+                                       0.70710678118654752440,
+                                       0.38268343236508977173,
+                                       0.0,
+                                       -0.38268343236508977173,
+                                       -0.70710678118654752440,
+                                       -0.92387953251128675613};
+__device__ constexpr double kS16[8] = {0.0,
+                                       0.38268343236508977173,
+                                       0.70710678118654752440,
+                                       0.92387953251128675613,
+                                       1.0,
+                                       0.92387953251128675613,
+                                       0.70710678118654752440,
+                                       0.38268343236508977173};
+
+// x * w_N^k, w_N = exp(-+2 pi i / N), N <= 16, k < N / 2
+template <typename R, int N, int K, bool INV, typename C>
+__device__ __forceinline__ C wmul(C x) {
+  constexpr int q = K * (16 / N);
+  if constexpr (q == 0) {
+    return x;
+  } else if constexpr (q == 4) {  // -i (forward) or +i (inverse)
+    return INV ? C{-x.y, x.x} : C{x.y, -x.x};
+  } else {
+    const R c = (R)kC16[q], s = (R)kS16[q];
+    return tmul<INV>(x, C{c, s});
+  }
+}
+
+template <typename R, int N, int K, bool INV, typename C>
+__device__ __forceinline__ void dft_combine(C* v, const C* e, const C* o) {
+  if constexpr (K < N / 2) {
+    const C t = wmul<R, N, K, INV>(o[K]);
+    v[K] = cadd(e[K], t);
+    v[K + N / 2] = csub(e[K], t);
+    dft_combine<R, N, K + 1, INV>(v, e, o);
+  }
+}
+
+// In-register DFT of N <= 16 points in natural order (radix-2 recursion).
+template <typename R, int N, bool INV, typename C>
+__device__ __forceinline__ void dft(C* v) {
+  if constexpr (N > 1) {
+    C e[N / 2], o[N / 2];
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+      e[k] = v[2 * k];
+      o[k] = v[2 * k + 1];
+    }
+    dft<R, N / 2, INV>(e);
+    dft<R, N / 2, INV>(o);
+    dft_combine<R, N, 0, INV>(v, e, o);
+  }
+}
+
+// One Stockham pass of radix RAD over a line of L points in LDS, stage NS
+// (the product of the radices of the previous passes): butterfly j reads
+// line[j + r L / RAD], applies w_{NS RAD}^{(j mod NS) r}, transforms, and
+// writes line[(j / NS) NS RAD + j mod NS + r NS].  Thread j0 of the line
+// does butterflies j0 + b L / 16, b < 16 / RAD.
+template <typename R, int L, int RAD, int NS, bool INV, typename C>
+__device__ __forceinline__ void stockham(C* line, int j0, const C* tw) {
+  constexpr int TL = L / 16, B = 16 / RAD;
+  C v[B][RAD];
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = j0 + b * TL;
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[b][r] = line[j + r * (L / RAD)];
+    if constexpr (NS > 1) {
+      const int jm = j % NS;
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) v[b][r] = tmul<INV>(v[b][r], tw[jm * r * (L / (NS * RAD))]);
+    }
+    dft<R, RAD, INV>(v[b]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int j = j0 + b * TL;
+    const int o = (j / NS) * NS * RAD + (j % NS);
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) line[o + r * NS] = v[b][r];
+  }
+  __syncthreads();
+}
+
+template <typename R, int L, int NS, bool INV, typename C>
+__device__ __forceinline__ void passes16(C* line, int j0, const C* tw) {
+  if constexpr (NS < L) {
+    stockham<R, L, 16, NS, INV>(line, j0, tw);
+    passes16<R, L, NS * 16, INV>(line, j0, tw);
+  }
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+// FFT of the line (all threads of the workgroup call this together).
+template <typename R, int L, bool INV, typename C>
+__device__ __forceinline__ void line_fft(C* line, int j0, const C* tw) {
+  constexpr int R0 = 1 << (ilog2(L) % 4);
+  if constexpr (R0 > 1) {
+    stockham<R, L, R0, 1, INV>(line, j0, tw);
+    passes16<R, L, R0, INV>(line, j0, tw);
+  } else {
+    passes16<R, L, 1, INV>(line, j0, tw);
+  }
+}
+
+// Grid position of image index i (one dimension).
+__device__ __forceinline__ int grid_pos(int i, int N, int Nt, int Na) { return i < Na ? i - Na + Nt : i - Na; }
+// Image index of grid position g, or -1 if g is not an image position.
+__device__ __forceinline__ int image_idx(int g, int N, int Nt, int Na) {
+  if (g < N - Na) return g + Na;
+  if (g >= Nt - Na) return g - (Nt - Na);
+  return -1;
+}
+
+template <typename R, int L>
+struct LineGeo {
+  static constexpr int E = sizeof(R) == 4 ? 4096 : 2048;  // points per workgroup
+  static constexpr int NT = E / 16;                        // threads per workgroup
+  static constexpr int TL = L / 16;                        // threads per line
+  static constexpr int LPB = E / L;                        // lines per workgroup
+  static constexpr int LS = L + 1;                         // LDS line stride (elements)
+  static_assert(L >= 16 && L <= E, "line length");
+};
+
+template <int NT, typename C>
+__device__ __forceinline__ void load_tw(C* tw, const void* src, int L) {
+  for (int i = threadIdx.x; i < L; i += NT) tw[i] = ((const C*)src)[i];
+}
+
+// Deconvolution factor product of image line q = iz N1 + iy, as
+// deconvolve! forms it (oscale times the outer factors).
+__device__ __forceinline__ double line_factor(const FArgs& a, int iy, int iz, bool with_oscale) {
+  double f = with_oscale ? a.oscale : 1.0;
+  if (a.D >= 2) f *= a.fac[1][iy];
+  if (a.D >= 3) f *= a.fac[2][iz];
+  return f;
+}
+
+// Forward, dimension 1: deconvolution of f into lines (y, z) in S_y x S_z,
+// zero padding, FFT, whole lines to the grid.
+template <typename R, int L>
+__global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_fwd(FArgs a, const typename cpx<R>::T* __restrict__ f,
+                                                   typename cpx<R>::T* __restrict__ grid) {
+  using C = typename cpx<R>::T;
+  using G = LineGeo<R, L>;
+  __shared__ __align__(16) R sm[2 * (G::LPB * G::LS + L)];
+  C* buf = (C*)sm;
+  C* tw = buf + G::LPB * G::LS;
+  load_tw<G::NT>(tw, a.tw[0], L);
+  const int N0 = a.N[0], Na0 = a.Na[0], N1 = a.N[1];
+  const int nlines = a.N[1] * a.N[2];
+  const int l = threadIdx.x / G::TL, j0 = threadIdx.x % G::TL;
+  const int q = blockIdx.x * G::LPB + l;
+  C* line = buf + l * G::LS;
+  if (q < nlines) {
+    const int iy = q % N1, iz = q / N1;
+    const double fr = line_factor(a, iy, iz, true);
+    const C* src = f + (size_t)q * N0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int p = j0 + k * G::TL;
+      const int ix = image_idx(p, N0, L, Na0);
+      C v{0, 0};
+      if (ix >= 0) {
+        const C x = src[ix];
+        const double s = a.fac[0][ix] * fr;
+        v = C{(R)((double)x.x * s), (R)((double)x.y * s)};
+      }
+      line[p] = v;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) line[j0 + k * G::TL] = C{0, 0};
+  }
+  __syncthreads();
+  line_fft<R, L, false>(line, j0, tw);
+  if (q < nlines) {
+    const int iy = q % N1, iz = q / N1;
+    const size_t o = (size_t)grid_pos(iz, a.N[2], a.L[2], a.Na[2]) * a.gstr[2] +
+                     (size_t)grid_pos(iy, N1, a.L[1], a.Na[1]) * a.gstr[1];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int p = j0 + k * G::TL;
+      grid[o + p] = line[p];
+    }
+  }
+}
+
+// Adjoint, dimension 1: lines (y, z) in S_y x S_z of the grid, inverse FFT,
+// positions S_x to f with the transposed deconvolution.
+template <typename R, int L>
+__global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_adj(FArgs a, const typename cpx<R>::T* __restrict__ grid,
+                                                   typename cpx<R>::T* __restrict__ f) {
+  using C = typename cpx<R>::T;
+  using G = LineGeo<R, L>;
+  __shared__ __align__(16) R sm[2 * (G::LPB * G::LS + L)];
+  C* buf = (C*)sm;
+  C* tw = buf + G::LPB * G::LS;
+  load_tw<G::NT>(tw, a.tw[0], L);
+  const int N0 = a.N[0], Na0 = a.Na[0], N1 = a.N[1];
+  const int nlines = a.N[1] * a.N[2];
+  const int l = threadIdx.x / G::TL, j0 = threadIdx.x % G::TL;
+  const int q = blockIdx.x * G::LPB + l;
+  C* line = buf + l * G::LS;
+  if (q < nlines) {
+    const int iy = q % N1, iz = q / N1;
+    const size_t o = (size_t)grid_pos(iz, a.N[2], a.L[2], a.Na[2]) * a.gstr[2] +
+                     (size_t)grid_pos(iy, N1, a.L[1], a.Na[1]) * a.gstr[1];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int p = j0 + k * G::TL;
+      line[p] = grid[o + p];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) line[j0 + k * G::TL] = C{0, 0};
+  }
+  __syncthreads();
+  line_fft<R, L, true>(line, j0, tw);
+  if (q < nlines) {
+    const int iy = q % N1, iz = q / N1;
+    const double fr = line_factor(a, iy, iz, false);
+    C* dst = f + (size_t)q * N0;
+    for (int ix = j0; ix < N0; ix += G::TL) {
+      const C v = line[grid_pos(ix, N0, L, Na0)];
+      const double s = a.fac[0][ix] * fr * a.oscale;
+      dst[ix] = C{(R)((double)v.x * s), (R)((double)v.y * s)};
+    }
+  }
+}
+
+// Strided pass along dimension d (1 or 2) over lines (x, c): x in
+// [0, Nt_0) in chunks of LPB, c the position along the remaining dimension
+// o (all of [0, Nt_o) if call, else S_o).  MODE 1: input nonzero on S_d
+// only (forward); MODE 2: output kept on S_d only (adjoint).
+template <typename R, int L, bool INV, int MODE>
+__global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_col(FArgs a, typename cpx<R>::T* __restrict__ grid, int d, int call) {
+  using C = typename cpx<R>::T;
+  using G = LineGeo<R, L>;
+  __shared__ __align__(16) R sm[2 * (G::LPB * G::LS + L)];
+  C* buf = (C*)sm;
+  C* tw = buf + G::LPB * G::LS;
+  load_tw<G::NT>(tw, a.tw[d], L);
+  const int o = 3 - d;  // d = 1: o = 2 (Nt = 1 for D = 2); d = 2: o = 1
+  const int c = blockIdx.y;
+  const int gc = call ? c : grid_pos(c, a.N[o], a.L[o], a.Na[o]);
+  const int x0 = blockIdx.x * G::LPB;
+  const int L0 = a.L[0];
+  const size_t base = (size_t)gc * a.gstr[o] + x0;
+  const long long st = a.gstr[d];
+  const int Nd = a.N[d], Nad = a.Na[d];
+  if (MODE == 1) {
+    // S_d positions from the grid, the gap [Nd - Nad, L - Nad) zero
+    for (int e = threadIdx.x; e < G::LPB * Nd; e += G::NT) {
+      const int xl = e % G::LPB, i = e / G::LPB;
+      const int k = grid_pos(i, Nd, L, Nad);
+      buf[xl * G::LS + k] = x0 + xl < L0 ? grid[base + xl + (size_t)k * st] : C{0, 0};
+    }
+    for (int e = threadIdx.x; e < G::LPB * (L - Nd); e += G::NT) {
+      const int xl = e % G::LPB, k = Nd - Nad + e / G::LPB;
+      buf[xl * G::LS + k] = C{0, 0};
+    }
+  } else {
+    for (int e = threadIdx.x; e < G::LPB * L; e += G::NT) {
+      const int xl = e % G::LPB, k = e / G::LPB;
+      buf[xl * G::LS + k] = x0 + xl < L0 ? grid[base + xl + (size_t)k * st] : C{0, 0};
+    }
+  }
+  __syncthreads();
+  const int l = threadIdx.x / G::TL, j0 = threadIdx.x % G::TL;
+  line_fft<R, L, INV>(buf + l * G::LS, j0, tw);
+  if (MODE == 2) {
+    for (int e = threadIdx.x; e < G::LPB * Nd; e += G::NT) {
+      const int xl = e % G::LPB, i = e / G::LPB;
+      const int k = grid_pos(i, Nd, L, Nad);
+      if (x0 + xl < L0) grid[base + xl + (size_t)k * st] = buf[xl * G::LS + k];
+    }
+  } else {
+    for (int e = threadIdx.x; e < G::LPB * L; e += G::NT) {
+      const int xl = e % G::LPB, k = e / G::LPB;
+      if (x0 + xl < L0) grid[base + xl + (size_t)k * st] = buf[xl * G::LS + k];
+    }
+  }
+}
+
+template <typename R, int L>
+hipError_t x_pass(bool fwd, const FArgs& a, hipStream_t s, const void* in, void* out) {
+  using C = typename cpx<R>::T;
+  const int nlines = a.N[1] * a.N[2];
+  using G = LineGeo<R, L>;
+  const dim3 g((unsigned)((nlines + G::LPB - 1) / G::LPB)), b(G::NT);
+  if (fwd) hipLaunchKernelGGL((k_fft_x_fwd<R, L>), g, b, 0, s, a, (const C*)in, (C*)out);
+  else hipLaunchKernelGGL((k_fft_x_adj<R, L>), g, b, 0, s, a, (const C*)in, (C*)out);
+  return hipGetLastError();
+}
+
+template <typename R, int L>
+hipError_t col_pass(bool fwd, const FArgs& a, hipStream_t s, void* grid, int d) {
+  using C = typename cpx<R>::T;
+  const int o = 3 - d;
+  // lines along d: (x, z in S_z) for d = 1, (x, y) for d = 2
+  const int call = d == 2 ? 1 : 0;
+  const int nc = call ? a.L[o] : a.N[o];
+  using G = LineGeo<R, L>;
+  const dim3 g((unsigned)((a.L[0] + G::LPB - 1) / G::LPB), (unsigned)nc), b(G::NT);
+  if (fwd) hipLaunchKernelGGL((k_fft_col<R, L, false, 1>), g, b, 0, s, a, (C*)grid, d, call);
+  else hipLaunchKernelGGL((k_fft_col<R, L, true, 2>), g, b, 0, s, a, (C*)grid, d, call);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t x_pass_l(bool fwd, const FArgs& a, hipStream_t s, const void* in, void* out) {
+  switch (a.L[0]) {
+#define NFFTM_XL(LL) \
+  case LL:           \
+    return x_pass<R, LL>(fwd, a, s, in, out);
+    NFFTM_XL(16) NFFTM_XL(32) NFFTM_XL(64) NFFTM_XL(128) NFFTM_XL(256) NFFTM_XL(512) NFFTM_XL(1024)
+#undef NFFTM_XL
+    case 2048:
+      if constexpr (sizeof(R) == 4) return x_pass<R, 2048>(fwd, a, s, in, out);
+      return hipErrorInvalidValue;
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <typename R>
+hipError_t col_pass_l(bool fwd, const FArgs& a, hipStream_t s, void* grid, int d) {
+  switch (a.L[d]) {
+#define NFFTM_CL(LL) \
+  case LL:           \
+    return col_pass<R, LL>(fwd, a, s, grid, d);
+    NFFTM_CL(16) NFFTM_CL(32) NFFTM_CL(64) NFFTM_CL(128) NFFTM_CL(256) NFFTM_CL(512) NFFTM_CL(1024)
+#undef NFFTM_CL
+    case 2048:
+      if constexpr (sizeof(R) == 4) return col_pass<R, 2048>(fwd, a, s, grid, d);
+      return hipErrorInvalidValue;
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+bool pruned_fft_supported(int D, const int64_t* N, const int64_t* Nt, bool dbl) {
+  if (D < 1 || D > 3) return false;
+  long long q = 1;
+  for (int d = 0; d < D; ++d) {
+    const int64_t n = Nt[d];
+    if (n < 16 || n > (dbl ? 1024 : 2048) || (n & (n - 1)) || N[d] > n) return false;
+    q *= n;
+  }
+  return q < (1LL << 31);
+}
+
+// Forward: part 0 = dimension-1 pass from f (deconvolution fused), part 1 =
+// the passes along dimensions 2 and 3.
+hipError_t launch_pruned_fft_fwd(bool dbl, const FArgs& a, hipStream_t s, const void* f, void* grid, int part) {
+  if (part == 0) return dbl ? x_pass_l<double>(true, a, s, f, grid) : x_pass_l<float>(true, a, s, f, grid);
+  for (int d = 1; d < a.D; ++d) {
+    const hipError_t e = dbl ? col_pass_l<double>(true, a, s, grid, d) : col_pass_l<float>(true, a, s, grid, d);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Adjoint: part 0 = the passes along dimensions 3 and 2, part 1 = the
+// dimension-1 pass into f (transposed deconvolution fused).
+hipError_t launch_pruned_fft_adj(bool dbl, const FArgs& a, hipStream_t s, void* grid, void* f, int part) {
+  if (part == 1) return dbl ? x_pass_l<double>(false, a, s, grid, f) : x_pass_l<float>(false, a, s, grid, f);
+  for (int d = a.D - 1; d >= 1; --d) {
+    const hipError_t e = dbl ? col_pass_l<double>(false, a, s, grid, d) : col_pass_l<float>(false, a, s, grid, d);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace nfftm

@@ -20,6 +20,26 @@ struct DArgs {
   FastDiv div_nt[MAXD];    // Nt[d]
 };
 
+// Arguments of the pruned FFT passes (k_fft.hip): per dimension the image
+// size, the oversampled grid size (1 beyond D), Na = N / 2 of the image
+// placement, the grid stride, the deconvolution factors (FP64) and the
+// twiddle table exp(-2 pi i k / Nt) (cos, sin pairs of the data precision,
+// k < Nt).
+struct FArgs {
+  int D;
+  int N[3], L[3], Na[3];
+  long long gstr[3];
+  const double* fac[3];
+  const void* tw[3];
+  double oscale;
+};
+
+// Power-of-two grids within the line kernels' LDS (FP32: 16..2048 points
+// per dimension, FP64: 16..1024), D <= 3, N_d <= Nt_d.
+bool pruned_fft_supported(int D, const int64_t* N, const int64_t* Nt, bool dbl);
+hipError_t launch_pruned_fft_fwd(bool dbl, const FArgs& a, hipStream_t s, const void* f, void* grid, int part);
+hipError_t launch_pruned_fft_adj(bool dbl, const FArgs& a, hipStream_t s, void* grid, void* f, int part);
+
 // LDS bytes of a tile kernel.
 size_t tile_lds_bytes(int D, bool dbl, int W, int tileElems);
 

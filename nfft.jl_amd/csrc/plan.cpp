@@ -338,6 +338,10 @@ struct nfftm_plan_s {
   size_t lds = 0;
 
   std::unique_ptr<FFTStage> fft;
+  // pruned FFT passes (power-of-two grids): twiddle tables per dimension
+  bool pfft = false;
+  DevBuf<char> d_tw;
+  size_t tw_off[3] = {0, 0, 0};
   double timing[7] = {0};
   // kernel-level profiling (event pairs recorded without host sync)
   bool profiling = false;
@@ -457,6 +461,27 @@ struct nfftm_plan_s {
     a.scratch = d_scratch.p;
     a.tstart = d_tstart.p;
     a.ntile_all = ntiles[0] * ntiles[1] * ntiles[2];
+    return a;
+  }
+
+  FArgs fargs() const {
+    FArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.D = D;
+    i64 gs = 1, off = 0;
+    for (int d = 0; d < 3; ++d) {
+      a.N[d] = d < D ? (int)N[d] : 1;
+      a.L[d] = d < D ? (int)Nt[d] : 1;
+      a.Na[d] = d < D ? (int)(N[d] / 2) : 0;
+      a.gstr[d] = gs;
+      if (d < D) {
+        gs *= Nt[d];
+        a.fac[d] = d_deconv.p + off;
+        off += N[d];
+        a.tw[d] = d_tw.p + tw_off[d];
+      }
+    }
+    a.oscale = 1.0;
     return a;
   }
 
@@ -813,6 +838,51 @@ int run_deconv(nfftm_plan_s& p, bool fwd, const void* in, void* out, bool standa
   return NFFTM_OK;
 }
 
+// Forward deconvolution and FFT into the grid: stage 0 (deconvolve!, or the
+// pruned dimension-1 pass with the deconvolution fused) and stage 1 (the FFT,
+// or the remaining pruned passes), with an optional timer mark in between.
+template <typename Mark>
+int forward_to_grid(nfftm_plan_s& p, const void* f, Mark&& mark) {
+  if (p.pfft) {
+    const FArgs a = p.fargs();
+    {
+      ProfScope ps(p, 0);
+      HIPCHK(launch_pruned_fft_fwd(p.dbl(), a, p.stream, f, p.d_grid.p, 0));
+    }
+    if (int rc = mark()) return rc;
+    ProfScope ps(p, 1);
+    HIPCHK(launch_pruned_fft_fwd(p.dbl(), a, p.stream, f, p.d_grid.p, 1));
+    return NFFTM_OK;
+  }
+  if (int rc = run_deconv(p, true, f, p.d_grid.p)) return rc;
+  if (int rc = mark()) return rc;
+  ProfScope ps(p, 1);
+  return p.fft->exec(true, p.d_grid.p, p.celem(), p.stream);
+}
+
+// Adjoint inverse FFT of the grid and deconvolution into f: stage 5 and
+// stage 6, mark in between.
+template <typename Mark>
+int grid_to_image(nfftm_plan_s& p, void* f, Mark&& mark) {
+  if (p.pfft) {
+    const FArgs a = p.fargs();
+    {
+      ProfScope ps(p, 5);
+      HIPCHK(launch_pruned_fft_adj(p.dbl(), a, p.stream, p.d_grid.p, f, 0));
+    }
+    if (int rc = mark()) return rc;
+    ProfScope ps(p, 6);
+    HIPCHK(launch_pruned_fft_adj(p.dbl(), a, p.stream, p.d_grid.p, f, 1));
+    return NFFTM_OK;
+  }
+  {
+    ProfScope ps(p, 5);
+    if (int rc = p.fft->exec(false, p.d_grid.p, p.celem(), p.stream)) return rc;
+  }
+  if (int rc = mark()) return rc;
+  return run_deconv(p, false, p.d_grid.p, f);
+}
+
 struct Timer {
   nfftm_plan_s& p;
   bool on;
@@ -861,6 +931,35 @@ int create_impl(nfftm_plan_s& p, const double* k) {
   HIPCHK(p.d_grid.alloc((size_t)p.Q * p.celem()));
   p.fft.reset(new FFTStage());
   if (int rc = p.fft->build(p.D, p.Nt, p.dbl())) return rc;
+  p.pfft = !(p.flags & NFFTM_FLAG_ROCFFT) && pruned_fft_supported(p.D, p.N, p.Nt, p.dbl());
+  if (p.pfft) {
+    // twiddle tables exp(-2 pi i k / Nt_d) as (cos, sin) of 2 pi k / Nt_d,
+    // FP64 on the host, stored in the data precision
+    const size_t es = p.dbl() ? 16 : 8;
+    size_t bytes = 0;
+    for (int d = 0; d < p.D; ++d) {
+      p.tw_off[d] = bytes;
+      bytes += (size_t)p.Nt[d] * es;
+    }
+    std::vector<char> h(bytes);
+    for (int d = 0; d < p.D; ++d) {
+      const i64 L = p.Nt[d];
+      for (i64 k = 0; k < L; ++k) {
+        const double th = 2.0 * M_PI * (double)k / (double)L;
+        const double c = std::cos(th), sn = std::sin(th);
+        char* e = h.data() + p.tw_off[d] + (size_t)k * es;
+        if (p.dbl()) {
+          ((double*)e)[0] = c;
+          ((double*)e)[1] = sn;
+        } else {
+          ((float*)e)[0] = (float)c;
+          ((float*)e)[1] = (float)sn;
+        }
+      }
+    }
+    HIPCHK(p.d_tw.alloc(bytes));
+    HIPCHK(hipMemcpy(p.d_tw.p, h.data(), bytes, hipMemcpyHostToDevice));
+  }
   for (auto& e : p.ev) HIPCHK(hipEventCreate(&e));
   return NFFTM_OK;
 }
@@ -1301,15 +1400,14 @@ namespace {
 // Launch sequence of one direction on p.stream: forward = deconvolve, FFT,
 // interpolate; adjoint = spread (+ fold), backward FFT, deconvolve_transpose.
 int enqueue_transform(nfftm_plan_s& p, bool fwd, const void* in, void* out) {
+  auto none = [] { return (int)NFFTM_OK; };
   if (fwd) {
-    if (int rc = run_deconv(p, true, in, p.d_grid.p)) return rc;
-    if (int rc = p.fft->exec(true, p.d_grid.p, p.celem(), p.stream)) return rc;
+    if (int rc = forward_to_grid(p, in, none)) return rc;
     if (p.M) return run_conv(p, false, p.d_grid.p, out);
     return NFFTM_OK;
   }
   if (int rc = run_conv(p, true, in, p.d_grid.p)) return rc;
-  if (int rc = p.fft->exec(false, p.d_grid.p, p.celem(), p.stream)) return rc;
-  return run_deconv(p, false, p.d_grid.p, out);
+  return grid_to_image(p, out, none);
 }
 
 // Graph replay of enqueue_transform for a repeated (input, output) pair.
@@ -1364,12 +1462,7 @@ int nfftm_forward(nfftm_plan p, const void* f, void* fhat) {
     return graph_transform(*p, true, f, fhat);
   Timer t(*p);
   if (int rc = t.mark(0)) return rc;
-  if (int rc = run_deconv(*p, true, f, p->d_grid.p)) return rc;
-  if (int rc = t.mark(1)) return rc;
-  {
-    ProfScope ps(*p, 1);
-    if (int rc = p->fft->exec(true, p->d_grid.p, p->celem(), p->stream)) return rc;
-  }
+  if (int rc = forward_to_grid(*p, f, [&] { return t.mark(1); })) return rc;
   if (int rc = t.mark(2)) return rc;
   if (p->M) {
     if (int rc = run_conv(*p, false, p->d_grid.p, fhat)) return rc;
@@ -1388,12 +1481,7 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
   if (int rc = t.mark(0)) return rc;
   if (int rc = run_conv(*p, true, fhat, p->d_grid.p)) return rc;
   if (int rc = t.mark(1)) return rc;
-  {
-    ProfScope ps(*p, 5);
-    if (int rc = p->fft->exec(false, p->d_grid.p, p->celem(), p->stream)) return rc;
-  }
-  if (int rc = t.mark(2)) return rc;
-  if (int rc = run_deconv(*p, false, p->d_grid.p, f)) return rc;
+  if (int rc = grid_to_image(*p, f, [&] { return t.mark(2); })) return rc;
   if (int rc = t.mark(3)) return rc;
   return t.finish(&p->timing[4], &p->timing[5], &p->timing[6]);
 }
@@ -1402,11 +1490,7 @@ int nfftm_grad_nodes(nfftm_plan p, const void* f, const void* ybar, double* kbar
   g_err.clear();
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
   if ((!f && p->P) || (!ybar && p->M) || (!kbar && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
-  if (int rc = run_deconv(*p, true, f, p->d_grid.p)) return rc;
-  {
-    ProfScope ps(*p, 1);
-    if (int rc = p->fft->exec(true, p->d_grid.p, p->celem(), p->stream)) return rc;
-  }
+  if (int rc = forward_to_grid(*p, f, [] { return (int)NFFTM_OK; })) return rc;
   if (p->M) {
     ProfScope ps(*p, 7);
     HIPCHK(launch_grad_nodes(p->dbl(), p->W, p->kargs(), p->stream, p->d_grid.p, ybar, kbar));

@@ -255,7 +255,8 @@ class NFFTPlan:
         self._handle = ctypes.c_void_p()
         kj = np.ascontiguousarray(k[::-1].T, dtype=np.float64)
         Nj = (ctypes.c_int64 * D)(*N[::-1])
-        flags = (_L.FLAG_BLOCKING if blocking else 0) | (_L.FLAG_GRAPHS if kw.get("graphs", False) else 0)
+        flags = ((_L.FLAG_BLOCKING if blocking else 0) | (_L.FLAG_GRAPHS if kw.get("graphs", False) else 0)
+                 | (_L.FLAG_ROCFFT if kw.get("rocfft", False) else 0))
         _check(self._lib.nfftm_plan_create(
             ctypes.byref(self._handle), prec, D, Nj, k.shape[1],
             kj.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m, sigma, _WINDOWS[window],

@@ -1,0 +1,119 @@
+"""Pruned FFT passes (k_fft.hip) against the whole-grid rocFFT stage.
+
+On power-of-two grids a plan runs its FFT stage as pruned line passes with
+the deconvolution fused (the default); ``rocfft=True`` selects rocFFT over
+the whole oversampled grid with separate deconvolution passes.  Both must
+give the same transforms: FP64 rel-l2 <= 1e-13 between the two paths and
+<= 1e-12 against the CPU oracle; FP32 <= 2e-6 between the paths.  The cases
+cover D = 1, 2, 3, odd and even N (the fftshift split Na = N / 2 of
+src/deconvolution.jl:22-45), grids with different sizes per dimension, and
+line lengths from 16 to 1024 points.
+"""
+import numpy as np
+import pytest
+import torch
+
+import nfftm
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu().numpy()
+    if isinstance(b, torch.Tensor):
+        b = b.detach().cpu().numpy()
+    return O.relerr(a, b)
+
+
+def crandn(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# (N in C order, sigma); every oversampled size is a power of two
+CASES = [
+    ((100,), 2.56),            # Nt = 256, N even
+    ((61,), 256 / 61),         # Nt = 256, N odd
+    ((300,), 1024 / 300),      # Nt = 1024
+    ((32, 64), 2.0),           # Nt = (64, 128)
+    ((15, 30), 64 / 30),       # Nt = (32, 64), odd N
+    ((8, 16, 32), 2.0),        # Nt = (16, 32, 64)
+    ((16, 8, 31), 2.05),       # Nt = (32, 16, 64), odd N
+    ((64, 64, 64), 2.0),       # Nt = 128^3
+]
+
+
+def make(k, N, sigma, dtype, rocfft):
+    return nfftm.plan_nfft(k, N, m=4, sigma=sigma, precompute=nfftm.LINEAR, dtype=dtype, rocfft=rocfft)
+
+
+@pytest.mark.parametrize("N,sigma", CASES, ids=[f"{'x'.join(map(str, c[0]))}" for c in CASES])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_pruned_matches_rocfft(N, sigma, dtype):
+    print(f"pruned N={N} sigma={sigma:.4f} dtype={dtype}", flush=True)
+    rng = np.random.default_rng([len(N), sum(N), dtype == torch.float64])
+    M = 3000 if np.prod(N) < 1e5 else 40000
+    k = rng.random((len(N), M)) - 0.5
+    p = make(k, N, sigma, dtype, False)
+    q = make(k, N, sigma, dtype, True)
+    assert all(n & (n - 1) == 0 for n in p.Ntilde), p.Ntilde
+    cd = torch.complex128 if dtype == torch.float64 else torch.complex64
+    f = torch.as_tensor(crandn(rng, N), dtype=cd, device="cuda")
+    fh = torch.as_tensor(crandn(rng, M), dtype=cd, device="cuda")
+    tol = 1e-13 if dtype == torch.float64 else 2e-6
+    y, yq = p * f, q * f
+    x, xq = p.H * fh, q.H * fh
+    torch.cuda.synchronize()
+    assert rel(y, yq) < tol
+    assert rel(x, xq) < tol
+    if dtype == torch.float64 and np.prod(N) <= 4096:
+        orc = O.OraclePlan(k, N, m=4, sigma=sigma, precompute=O.LINEAR)
+        assert rel(y, orc.forward(f.cpu().numpy())) < 1e-12
+        assert rel(x, orc.adjoint(fh.cpu().numpy())) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_pruned_grad_and_repeat(dtype):
+    """Node gradient (forward FFT stage) on both paths, and repeated
+    transforms on one plan with other work in between (the pruned passes
+    leave grid regions unwritten that no later pass reads)."""
+    rng = np.random.default_rng(77)
+    N, M = (16, 32, 16), 5000
+    k = rng.random((3, M)) - 0.5
+    p = make(k, N, 2.0, dtype, False)
+    q = make(k, N, 2.0, dtype, True)
+    cd = torch.complex128 if dtype == torch.float64 else torch.complex64
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    for it in range(3):
+        f = torch.as_tensor(crandn(rng, N), dtype=cd, device="cuda")
+        fh = torch.as_tensor(crandn(rng, M), dtype=cd, device="cuda")
+        g1 = p.grad_nodes(f, fh)
+        g2 = q.grad_nodes(f, fh)
+        x = p.H * fh
+        _ = torch.randn(1 << 18, device="cuda").sum()
+        y = p * f
+        torch.cuda.synchronize()
+        assert rel(g1, g2) < tol, it
+        assert rel(x, q.H * fh) < tol, it
+        assert rel(y, q * f) < tol, it
+
+
+def test_pruned_timing_stages():
+    """TimingStats on the pruned path: the fused dimension-1 pass counts as
+    deconvolution, the other passes as the FFT."""
+    rng = np.random.default_rng(5)
+    N, M = (32, 32, 32), 20000
+    k = rng.random((3, M)) - 0.5
+    p = nfftm.plan_nfft(k, N, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=torch.float32)
+    q = nfftm.plan_nfft(k, N, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=torch.float32, rocfft=True)
+    f = torch.as_tensor(crandn(rng, N), dtype=torch.complex64, device="cuda")
+    fh = torch.as_tensor(crandn(rng, M), dtype=torch.complex64, device="cuda")
+    t = nfftm.TimingStats()
+    y = p.forward(f, timing=t)
+    x = p.adjoint(fh, timing=t)
+    torch.cuda.synchronize()
+    for key in ("conv", "fft", "deconv", "conv_adjoint", "fft_adjoint", "deconv_adjoint"):
+        assert getattr(t, key) > 0, (key, t)
+    assert rel(y, q * f) < 2e-6
+    assert rel(x, q.H * fh) < 2e-6
