@@ -215,17 +215,22 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_fwd(FArgs a, cons
     const int iy = q % N1, iz = q / N1;
     const double fr = line_factor(a, iy, iz, true);
     const C* src = f + (size_t)q * N0;
+    C x[16];
+    double fc[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const int p = j0 + k * G::TL;
-      const int ix = image_idx(p, N0, L, Na0);
-      C v{0, 0};
+      const int ix = image_idx(j0 + k * G::TL, N0, L, Na0);
+      x[k] = C{0, 0};
+      fc[k] = 0;
       if (ix >= 0) {
-        const C x = src[ix];
-        const double s = a.fac[0][ix] * fr;
-        v = C{(R)((double)x.x * s), (R)((double)x.y * s)};
+        x[k] = src[ix];
+        fc[k] = a.fac[0][ix];
       }
-      line[p] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const double s = fc[k] * fr;
+      line[j0 + k * G::TL] = C{(R)((double)x[k].x * s), (R)((double)x[k].y * s)};
     }
   } else {
 #pragma unroll
@@ -265,11 +270,11 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_adj(FArgs a, cons
     const int iy = q % N1, iz = q / N1;
     const size_t o = (size_t)grid_pos(iz, a.N[2], a.L[2], a.Na[2]) * a.gstr[2] +
                      (size_t)grid_pos(iy, N1, a.L[1], a.Na[1]) * a.gstr[1];
+    C x[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int p = j0 + k * G::TL;
-      line[p] = grid[o + p];
-    }
+    for (int k = 0; k < 16; ++k) x[k] = grid[o + j0 + k * G::TL];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) line[j0 + k * G::TL] = x[k];
   } else {
 #pragma unroll
     for (int k = 0; k < 16; ++k) line[j0 + k * G::TL] = C{0, 0};
@@ -308,21 +313,39 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_col(FArgs a, typena
   const size_t base = (size_t)gc * a.gstr[o] + x0;
   const long long st = a.gstr[d];
   const int Nd = a.N[d], Nad = a.Na[d];
+  // every load of a thread is issued before its LDS writes
+  constexpr int U = G::E / G::NT;
+  C v[U];
   if (MODE == 1) {
     // S_d positions from the grid, the gap [Nd - Nad, L - Nad) zero
-    for (int e = threadIdx.x; e < G::LPB * Nd; e += G::NT) {
+    const int cnt = G::LPB * Nd;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = threadIdx.x + u * G::NT;
       const int xl = e % G::LPB, i = e / G::LPB;
-      const int k = grid_pos(i, Nd, L, Nad);
-      buf[xl * G::LS + k] = x0 + xl < L0 ? grid[base + xl + (size_t)k * st] : C{0, 0};
+      if (e < cnt) v[u] = x0 + xl < L0 ? grid[base + xl + (size_t)grid_pos(i, Nd, L, Nad) * st] : C{0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = threadIdx.x + u * G::NT;
+      const int xl = e % G::LPB, i = e / G::LPB;
+      if (e < cnt) buf[xl * G::LS + grid_pos(i, Nd, L, Nad)] = v[u];
     }
     for (int e = threadIdx.x; e < G::LPB * (L - Nd); e += G::NT) {
       const int xl = e % G::LPB, k = Nd - Nad + e / G::LPB;
       buf[xl * G::LS + k] = C{0, 0};
     }
   } else {
-    for (int e = threadIdx.x; e < G::LPB * L; e += G::NT) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = threadIdx.x + u * G::NT;
       const int xl = e % G::LPB, k = e / G::LPB;
-      buf[xl * G::LS + k] = x0 + xl < L0 ? grid[base + xl + (size_t)k * st] : C{0, 0};
+      v[u] = x0 + xl < L0 ? grid[base + xl + (size_t)k * st] : C{0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = threadIdx.x + u * G::NT;
+      buf[(e % G::LPB) * G::LS + e / G::LPB] = v[u];
     }
   }
   __syncthreads();
