@@ -126,22 +126,27 @@ __device__ __forceinline__ void stage3(const C* __restrict__ grid, C* tile, int 
   const bool lx = px < P;
   const int gx = wrap1(x0 + (lx ? px : 0), Nx);
   for (int q0 = 0; q0 < ROWS; q0 += RB * RPP) {
-    C v[RB];
+    // plain scalars (not an array of vector structs) so that they stay in
+    // registers across the predicated loads
+    decltype(grid[0].x) vr[RB], vi[RB];
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
       const int r = q0 + b * RPP + r0;
-      if (lx && r < ROWS) {
-        const int py = r % P, pz = r / P;
-        const int gy = wrap1(y0 + py, Ny), gz = wrap1(z0 + pz, Nz);
-        const size_t g = off32 ? (size_t)((unsigned)gz * (unsigned)sz + (unsigned)gy * (unsigned)sy + (unsigned)gx)
-                               : (size_t)(gz * sz + gy * sy + gx);
-        v[b] = grid[g];
-      }
+      const bool ok = lx && r < ROWS;
+      const int rr = ok ? r : 0;
+      const int py = rr % P, pz = rr / P;
+      const int gy = wrap1(y0 + py, Ny), gz = wrap1(z0 + pz, Nz);
+      const size_t g = off32 ? (size_t)((unsigned)gz * (unsigned)sz + (unsigned)gy * (unsigned)sy + (unsigned)gx)
+                             : (size_t)(gz * sz + gy * sy + gx);
+      C v{0, 0};
+      if (ok) v = grid[g];
+      vr[b] = v.x;
+      vi[b] = v.y;
     }
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
       const int r = q0 + b * RPP + r0;
-      if (lx && r < ROWS) tile[px + G::SX * (r % P) + G::SXY * (r / P)] = v[b];
+      if (lx && r < ROWS) tile[px + G::SX * (r % P) + G::SXY * (r / P)] = C{vr[b], vi[b]};
     }
   }
 }
@@ -158,14 +163,20 @@ struct RecStage {
   static constexpr int N16 = G::NB * G::RB / 16;
   static constexpr int PER = (N16 + NT - 1) / NT;
   static_assert(G::RB % 16 == 0, "records are 16-byte blocks");
-  uint4 v[PER];
+  // scalar arrays (not an array of vector structs): kept in registers
+  unsigned v0[PER], v1[PER], v2[PER], v3[PER];
   __device__ __forceinline__ void load(const KArgs& a, int b0, int nb) {
     const uint4* src = (const uint4*)(a.rec + (size_t)b0 * G::RB);
     const int n16 = nb * (G::RB / 16);
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int i = (int)threadIdx.x + u * NT;
-      if (i < n16) v[u] = src[i];
+      uint4 t = make_uint4(0, 0, 0, 0);
+      if (i < n16) t = src[i];
+      v0[u] = t.x;
+      v1[u] = t.y;
+      v2[u] = t.z;
+      v3[u] = t.w;
     }
   }
   __device__ __forceinline__ void store(char* lrec, int nb) const {
@@ -174,7 +185,7 @@ struct RecStage {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int i = (int)threadIdx.x + u * NT;
-      if (i < n16) dst[i] = v[u];
+      if (i < n16) dst[i] = make_uint4(v0[u], v1[u], v2[u], v3[u]);
     }
   }
 };
@@ -182,7 +193,7 @@ struct RecStage {
 // V = 1 geometries hold two workgroups per CU: 2 NT / 256 waves per SIMD
 // (1024 threads: 8 waves, at most 64 VGPRs) where that needs no spills.
 template <typename R, int W, int NT, int V>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(V == 1 && W <= 12 ? NT / 128 : 1)))
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(V == 1 && W <= (sizeof(R) == 4 ? 12 : 8) ? NT / 128 : 1)))
 void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
                                                 typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
@@ -257,7 +268,7 @@ void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
 // scale from a strict bound; FP64: two FP64 planes); the padded tile goes to
 // the item's scratch slot.
 template <typename R, int W, int NT, int V>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(V == 1 && W <= 12 ? NT / 128 : 1)))
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(V == 1 && W <= (sizeof(R) == 4 ? 12 : 8) ? NT / 128 : 1)))
 void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
   using G = Geo3<R, W, V>;
@@ -277,11 +288,14 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
   rs.load(a, it.y, min(G::NB, it.z - it.y));
   constexpr int HPT = ITEM3 / NT;
   static_assert(ITEM3 % NT == 0, "item size");
-  C held[HPT];
+  R hre[HPT], him[HPT];
 #pragma unroll
   for (int h = 0; h < HPT; ++h) {
     const int j = it.y + (int)threadIdx.x + h * NT;
-    held[h] = j < it.z ? fhat[a.perm[j]] : C{0, 0};
+    C v{0, 0};
+    if (j < it.z) v = fhat[a.perm[j]];
+    hre[h] = v.x;
+    him[h] = v.y;
   }
 
   // FP32: fixed-point scale s = 2^(30 - e), B = sum_j |f_j|_inf * wmaxD < 2^e
@@ -289,7 +303,7 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
   if (sizeof(R) == 4) {
     R part = 0;
 #pragma unroll
-    for (int h = 0; h < HPT; ++h) part += fmax(fabs(held[h].x), fabs(held[h].y));
+    for (int h = 0; h < HPT; ++h) part += fmax(fabs(hre[h]), fabs(him[h]));
     part = wave_total(part);
     R* red = (R*)lrec;
     if (lane == 63) red[wave] = part;
@@ -312,7 +326,7 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
 #pragma unroll
     for (int h = 0; h < HPT; ++h) {
       const int n = it.y + (int)threadIdx.x + h * NT - b0;
-      if (n >= 0 && n < nb) fv[n] = C{held[h].x * sc, held[h].y * sc};
+      if (n >= 0 && n < nb) fv[n] = C{hre[h] * sc, him[h] * sc};
     }
     __syncthreads();
     if (b0 + G::NB < it.z) rs.load(a, b0 + G::NB, min(G::NB, it.z - b0 - G::NB));
