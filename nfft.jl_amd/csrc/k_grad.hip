@@ -85,6 +85,32 @@ __device__ __forceinline__ int grad_weights(const KArgs& a, int d, double ksv, d
   return (int)(c >= a.Nt[d] ? c - a.Nt[d] : c);
 }
 
+// Row of 2m grid points along dimension 1 at offset ofs: dot products with
+// w_0 (s) and dw_0 (t).
+template <typename R, int W, typename C>
+__device__ __forceinline__ void grad_row(const C* __restrict__ grid, long long ofs, const int* x0, const R* w0,
+                                         const R* dw0, R& sr, R& si, R& tr, R& ti) {
+  sr = si = tr = ti = 0;
+#pragma unroll
+  for (int q = 0; q < W; ++q) {
+    const C v = grid[ofs + x0[q]];
+    sr = fma(w0[q], v.x, sr);
+    si = fma(w0[q], v.y, si);
+    tr = fma(dw0[q], v.x, tr);
+    ti = fma(dw0[q], v.y, ti);
+  }
+}
+
+// v[i] for a runtime i of a register array (a select chain: no private
+// memory for the array).
+template <int W, typename R>
+__device__ __forceinline__ R pick(const R* v, int i) {
+  R r = v[0];
+#pragma unroll
+  for (int l = 1; l < W; ++l) r = i == l ? v[l] : r;
+  return r;
+}
+
 template <typename R, int W, int DD>
 __global__ __launch_bounds__(256) void k_grad_nodes(KArgs a, const typename cpx<R>::T* __restrict__ grid,
                                                     const typename cpx<R>::T* __restrict__ ybar,
@@ -96,7 +122,9 @@ __global__ __launch_bounds__(256) void k_grad_nodes(KArgs a, const typename cpx<
   if (s >= a.M) return;
   R w[DM][W], dw[DM][W];
   long long base[DM];
-  for (int d = 0; d < D; ++d) {
+#pragma unroll
+  for (int d = 0; d < DM; ++d) {
+    if (d >= D) break;
     double wd[W], dwd[W];
     const int cw = grad_weights<W>(a, d, a.ks[(size_t)d * a.M + s], wd, dwd);
 #pragma unroll
@@ -110,43 +138,83 @@ __global__ __launch_bounds__(256) void k_grad_nodes(KArgs a, const typename cpx<
 #pragma unroll
   for (int l = 0; l < W; ++l) x0[l] = (int)wrapidx(base[0] + l, a.Nt[0]);
   R gr[DM], gi[DM];
-  for (int d = 0; d < DM; ++d) gr[d] = gi[d] = 0;
-  int l[DM];
-  for (int d = 0; d < DM; ++d) l[d] = 0;
-  while (true) {
-    long long ofs = 0;
-    for (int d = 1; d < D; ++d) ofs += wrapidx(base[d] + l[d], a.Nt[d]) * a.gstr[d];
-    R sr = 0, si = 0, tr = 0, ti = 0;
 #pragma unroll
-    for (int q = 0; q < W; ++q) {
-      const C v = grid[ofs + x0[q]];
-      sr = fma(w[0][q], v.x, sr);
-      si = fma(w[0][q], v.y, si);
-      tr = fma(dw[0][q], v.x, tr);
-      ti = fma(dw[0][q], v.y, ti);
+  for (int d = 0; d < DM; ++d) gr[d] = gi[d] = 0;
+  if constexpr (DD == 1) {
+    R sr, si;
+    grad_row<R, W>(grid, 0, x0, w[0], dw[0], sr, si, gr[0], gi[0]);
+  } else if constexpr (DD == 2) {
+    // rows l_1: d/dk_0 takes (dw_0 . row) w_1, d/dk_1 takes (w_0 . row) dw_1
+#pragma unroll 1
+    for (int ly = 0; ly < W; ++ly) {
+      R sr, si, tr, ti;
+      grad_row<R, W>(grid, wrapidx(base[1] + ly, a.Nt[1]) * a.gstr[1], x0, w[0], dw[0], sr, si, tr, ti);
+      const R w1 = pick<W>(w[1], ly), dw1 = pick<W>(dw[1], ly);
+      gr[0] = fma(tr, w1, gr[0]);
+      gi[0] = fma(ti, w1, gi[0]);
+      gr[1] = fma(sr, dw1, gr[1]);
+      gi[1] = fma(si, dw1, gi[1]);
     }
-    R wo = 1;
-    for (int d = 1; d < D; ++d) wo *= w[d][l[d]];
-    gr[0] = fma(tr, wo, gr[0]);
-    gi[0] = fma(ti, wo, gi[0]);
-    for (int d = 1; d < D; ++d) {
-      R pw = dw[d][l[d]];
-      for (int e = 1; e < D; ++e)
-        if (e != d) pw *= w[e][l[e]];
-      gr[d] = fma(sr, pw, gr[d]);
-      gi[d] = fma(si, pw, gi[d]);
+  } else if constexpr (DD == 3) {
+    // planes l_2 of rows l_1: plane sums A = sum (dw_0 . row) w_1,
+    // B = sum (w_0 . row) dw_1, E = sum (w_0 . row) w_1, then weighted by
+    // w_2 (A, B) and dw_2 (E)
+#pragma unroll 1
+    for (int lz = 0; lz < W; ++lz) {
+      const long long oz = wrapidx(base[2] + lz, a.Nt[2]) * a.gstr[2];
+      R Ar = 0, Ai = 0, Br = 0, Bi = 0, Er = 0, Ei = 0;
+#pragma unroll
+      for (int ly = 0; ly < W; ++ly) {
+        R sr, si, tr, ti;
+        grad_row<R, W>(grid, oz + wrapidx(base[1] + ly, a.Nt[1]) * a.gstr[1], x0, w[0], dw[0], sr, si, tr, ti);
+        Ar = fma(tr, w[1][ly], Ar);
+        Ai = fma(ti, w[1][ly], Ai);
+        Br = fma(sr, dw[1][ly], Br);
+        Bi = fma(si, dw[1][ly], Bi);
+        Er = fma(sr, w[1][ly], Er);
+        Ei = fma(si, w[1][ly], Ei);
+      }
+      const R w2 = pick<W>(w[2], lz), dw2 = pick<W>(dw[2], lz);
+      gr[0] = fma(Ar, w2, gr[0]);
+      gi[0] = fma(Ai, w2, gi[0]);
+      gr[1] = fma(Br, w2, gr[1]);
+      gi[1] = fma(Bi, w2, gi[1]);
+      gr[2] = fma(Er, dw2, gr[2]);
+      gi[2] = fma(Ei, dw2, gi[2]);
     }
-    int d = 1;
-    for (; d < D; ++d) {
-      if (++l[d] < W) break;
-      l[d] = 0;
+  } else {
+    // any D: odometer over the outer indices
+    int l[DM];
+    for (int d = 0; d < DM; ++d) l[d] = 0;
+    while (true) {
+      long long ofs = 0;
+      for (int d = 1; d < D; ++d) ofs += wrapidx(base[d] + l[d], a.Nt[d]) * a.gstr[d];
+      R sr, si, tr, ti;
+      grad_row<R, W>(grid, ofs, x0, w[0], dw[0], sr, si, tr, ti);
+      R wo = 1;
+      for (int d = 1; d < D; ++d) wo *= w[d][l[d]];
+      gr[0] = fma(tr, wo, gr[0]);
+      gi[0] = fma(ti, wo, gi[0]);
+      for (int d = 1; d < D; ++d) {
+        R pw = dw[d][l[d]];
+        for (int e = 1; e < D; ++e)
+          if (e != d) pw *= w[e][l[e]];
+        gr[d] = fma(sr, pw, gr[d]);
+        gi[d] = fma(si, pw, gi[d]);
+      }
+      int d = 1;
+      for (; d < D; ++d) {
+        if (++l[d] < W) break;
+        l[d] = 0;
+      }
+      if (d >= D) break;
     }
-    if (d >= D) break;
   }
   const int jj = a.perm[s];
   const C y = ybar[jj];
-  for (int d = 0; d < D; ++d)
-    kbar[(size_t)d + (size_t)D * jj] = (double)y.x * (double)gr[d] + (double)y.y * (double)gi[d];
+#pragma unroll
+  for (int d = 0; d < DM; ++d)
+    if (d < D) kbar[(size_t)d + (size_t)D * jj] = (double)y.x * (double)gr[d] + (double)y.y * (double)gi[d];
 }
 
 template <typename R, int W>
