@@ -194,6 +194,38 @@ int nfftm_batch_forward(nfftm_batch b, const void* const* f_dev, void* const* fh
 int nfftm_batch_adjoint(nfftm_batch b, const void* const* fhat_dev, void* const* f_dev, void* hip_stream);
 void nfftm_batch_destroy(nfftm_batch b);
 
+/* NFFTTools on the plan.
+ *
+ * Sampling density compensation (Pipe & Menon): replaces
+ * sdc!(p, iters, weights, weights_tmp, workg[, workf, workv]) of
+ * NFFTTools/src/samplingDensity.jl:93-155 (reached from sdc(p; iters) at
+ * :59-86).  weights_dev holds M real values of the plan's precision (the
+ * initial weights, ones in sdc(p)) and receives the result.  Each iteration
+ * spreads the weights (convolve_transpose!), divides the grid by its maximum
+ * of the first iteration, interpolates (convolve!) and divides the weights by
+ * the result.  scale = 1 adds the final least-squares scale c with
+ * A' D(w) A 1 ~ 1 (the 7-argument sdc!); scale = 0 stops before it (the
+ * 5-argument sdc!).  A non-positive interpolated weight fails with
+ * NFFTM_ERR_UNSUPPORTED ("non-positive weights"), as the reference throws.
+ * Synchronises the plan's stream. */
+int nfftm_sdc(nfftm_plan p, int iters, void* weights_dev, int scale);
+
+/* Toeplitz embedding of A'A for an image of size N (D <= 3): an object
+ * owning the FFTs of size 2N and one work array of that size.
+ * nfftm_toeplitz_kernel: lambda = FFT(fftshift(adjoint(p) * ones(M))) for a
+ * plan p over 2N, as calculateToeplitzKernel!(f, p, tr, fftplan)
+ * (NFFTTools/src/Toeplitz.jl:119-125) after nodes!(p, tr).
+ * nfftm_toeplitz_apply: y <- the N-corner of IFFT(FFT(pad(y)) .* lambda), as
+ * convolveToeplitzKernel!(y, lambda, ...) (NFFTTools/src/Toeplitz.jl:236-250).
+ * lambda_dev and y_dev are device buffers of complex values of the
+ * precision (column-major, sizes 2N and N).  Work is enqueued on the
+ * object's stream (nfftm_toeplitz_kernel: on the plan's stream). */
+typedef struct nfftm_toeplitz_s* nfftm_toeplitz;
+int nfftm_toeplitz_create(nfftm_toeplitz* out, int prec, int D, const int64_t* N, void* hip_stream);
+int nfftm_toeplitz_kernel(nfftm_toeplitz t, nfftm_plan p, void* lambda_dev);
+int nfftm_toeplitz_apply(nfftm_toeplitz t, const void* lambda_dev, void* y_dev);
+void nfftm_toeplitz_destroy(nfftm_toeplitz t);
+
 /* Change the HIP stream the plan enqueues on. */
 int nfftm_set_stream(nfftm_plan p, void* hip_stream);
 

@@ -122,9 +122,9 @@ __global__ __launch_bounds__(256) void k_grad_nodes(KArgs a, const typename cpx<
   if (s >= a.M) return;
   R w[DM][W], dw[DM][W];
   long long base[DM];
-#pragma unroll
-  for (int d = 0; d < DM; ++d) {
-    if (d >= D) break;
+  // compile-time dimension loop for D <= 3 (register arrays), runtime beyond
+#pragma unroll(DD ? DD : 1)
+  for (int d = 0; d < D; ++d) {
     double wd[W], dwd[W];
     const int cw = grad_weights<W>(a, d, a.ks[(size_t)d * a.M + s], wd, dwd);
 #pragma unroll
@@ -163,16 +163,17 @@ __global__ __launch_bounds__(256) void k_grad_nodes(KArgs a, const typename cpx<
     for (int lz = 0; lz < W; ++lz) {
       const long long oz = wrapidx(base[2] + lz, a.Nt[2]) * a.gstr[2];
       R Ar = 0, Ai = 0, Br = 0, Bi = 0, Er = 0, Ei = 0;
-#pragma unroll
+#pragma unroll 1
       for (int ly = 0; ly < W; ++ly) {
         R sr, si, tr, ti;
         grad_row<R, W>(grid, oz + wrapidx(base[1] + ly, a.Nt[1]) * a.gstr[1], x0, w[0], dw[0], sr, si, tr, ti);
-        Ar = fma(tr, w[1][ly], Ar);
-        Ai = fma(ti, w[1][ly], Ai);
-        Br = fma(sr, dw[1][ly], Br);
-        Bi = fma(si, dw[1][ly], Bi);
-        Er = fma(sr, w[1][ly], Er);
-        Ei = fma(si, w[1][ly], Ei);
+        const R w1 = pick<W>(w[1], ly), dw1 = pick<W>(dw[1], ly);
+        Ar = fma(tr, w1, Ar);
+        Ai = fma(ti, w1, Ai);
+        Br = fma(sr, dw1, Br);
+        Bi = fma(si, dw1, Bi);
+        Er = fma(sr, w1, Er);
+        Ei = fma(si, w1, Ei);
       }
       const R w2 = pick<W>(w[2], lz), dw2 = pick<W>(dw[2], lz);
       gr[0] = fma(Ar, w2, gr[0]);

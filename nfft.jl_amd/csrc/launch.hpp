@@ -40,6 +40,30 @@ bool pruned_fft_supported(int D, const int64_t* N, const int64_t* Nt, bool dbl);
 hipError_t launch_pruned_fft_fwd(bool dbl, const FArgs& a, hipStream_t s, const void* f, void* grid, int part);
 hipError_t launch_pruned_fft_adj(bool dbl, const FArgs& a, hipStream_t s, void* grid, void* f, int part);
 
+// NFFTTools callers (k_tools.hip): reductions over complex vectors into a
+// device scalar, the sdc elementwise steps, and the Toeplitz pad / product /
+// crop of an image of size N in an oversampled array of size L (column-major,
+// D <= 3).
+enum { RED_MAX_RE = 0, RED_MIN_RE = 1, RED_SUM_RE = 2, RED_SUM_ABS2 = 3 };
+struct ToepArgs {
+  int D;
+  long long N[3], L[3];
+  long long P, Q;  // prod N, prod L
+};
+// part: at least 2048 doubles
+hipError_t launch_reduce(bool dbl, int op, const void* x, long long n, double* part, double* out, hipStream_t s);
+hipError_t launch_div_scalar(bool dbl, void* g, long long n, const double* sc, hipStream_t s);
+hipError_t launch_sdc_update(bool dbl, void* wc, const void* wt, long long n, const double* sc, int* bad,
+                             hipStream_t s);
+hipError_t launch_real_to_cpx(bool dbl, const void* w, void* wc, long long n, hipStream_t s);
+hipError_t launch_cpx_to_real(bool dbl, const void* wc, void* w, long long n, const double* sc, hipStream_t s);
+hipError_t launch_fill_one(bool dbl, void* u, long long n, hipStream_t s);
+hipError_t launch_mul_real(bool dbl, void* f, const void* wc, long long n, hipStream_t s);
+hipError_t launch_pad_corner(bool dbl, const void* y, void* x, const ToepArgs& a, hipStream_t s);
+hipError_t launch_crop_corner(bool dbl, const void* x, void* y, const ToepArgs& a, double sc, hipStream_t s);
+hipError_t launch_cmul(bool dbl, void* x, const void* lam, long long n, hipStream_t s);
+hipError_t launch_shift_sign(bool dbl, void* x, const ToepArgs& a, hipStream_t s);
+
 // LDS bytes of a tile kernel.
 size_t tile_lds_bytes(int D, bool dbl, int W, int tileElems);
 

@@ -1687,4 +1687,148 @@ void nfftm_batch_destroy(nfftm_batch b) {
   delete b;
 }
 
+// ---------------------------------------------------------------------------
+// NFFTTools: sampling density compensation and the Toeplitz embedding.
+// ---------------------------------------------------------------------------
+int nfftm_sdc(nfftm_plan p, int iters, void* weights, int scale) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if (iters < 0) return fail(NFFTM_ERR_ARG, "iters = %d < 0", iters);
+  if (!weights && p->M) return fail(NFFTM_ERR_ARG, "NULL weights");
+  if (p->M == 0) return NFFTM_OK;
+  const bool dbl = p->dbl();
+  const size_t ce = p->celem();
+  const i64 M = p->M;
+  hipStream_t s = p->stream;
+  DevBuf<char> wc, wt;
+  DevBuf<double> sc;  // [0] first grid maximum, [1] sum Re v, [2] sum |v|^2, then 2048 partials
+  DevBuf<int> bad;
+  HIPCHK(wc.alloc((size_t)M * ce));
+  HIPCHK(wt.alloc((size_t)M * ce));
+  HIPCHK(sc.alloc(8 + 2048));
+  HIPCHK(bad.alloc(1));
+  HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+  double* part = sc.p + 8;
+  HIPCHK(launch_real_to_cpx(dbl, weights, wc.p, M, s));
+  // sdc!(p, iters, weights, weights_tmp, workg), samplingDensity.jl:93-118
+  for (int i = 0; i < iters; ++i) {
+    if (int rc = run_conv(*p, true, wc.p, p->d_grid.p, true)) return rc;
+    if (i == 0) HIPCHK(launch_reduce(dbl, RED_MAX_RE, p->d_grid.p, p->Q, part, sc.p, s));
+    HIPCHK(launch_div_scalar(dbl, p->d_grid.p, p->Q, sc.p, s));
+    if (int rc = run_conv(*p, false, p->d_grid.p, wt.p, true)) return rc;
+    HIPCHK(launch_sdc_update(dbl, wc.p, wt.p, M, sc.p, bad.p, s));
+  }
+  if (scale) {
+    // c = real(sum(v)) / sum(abs2, v), v = A' D(w) A 1 (samplingDensity.jl:141-154)
+    DevBuf<char> u, f;
+    HIPCHK(u.alloc((size_t)p->P * ce));
+    HIPCHK(f.alloc((size_t)M * ce));
+    HIPCHK(launch_fill_one(dbl, u.p, p->P, s));
+    if (int rc = enqueue_transform(*p, true, u.p, f.p)) return rc;
+    HIPCHK(launch_mul_real(dbl, f.p, wc.p, M, s));
+    if (int rc = enqueue_transform(*p, false, f.p, u.p)) return rc;
+    HIPCHK(launch_reduce(dbl, RED_SUM_RE, u.p, p->P, part, sc.p + 1, s));
+    HIPCHK(launch_reduce(dbl, RED_SUM_ABS2, u.p, p->P, part, sc.p + 2, s));
+    HIPCHK(launch_cpx_to_real(dbl, wc.p, weights, M, sc.p + 1, s));
+    HIPCHK(hipStreamSynchronize(s));
+  } else {
+    HIPCHK(launch_cpx_to_real(dbl, wc.p, weights, M, nullptr, s));
+  }
+  int hbad = 0;
+  HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hbad) return fail(NFFTM_ERR_UNSUPPORTED, "non-positive weights");
+  return NFFTM_OK;
+}
+
+}  // extern "C"
+
+struct nfftm_toeplitz_s {
+  int prec = NFFTM_F32, D = 1;
+  i64 N[3] = {1, 1, 1}, L[3] = {1, 1, 1}, P = 1, Q = 1;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<FFTStage> fft;
+  DevBuf<char> x;
+  bool dbl() const { return prec == NFFTM_F64; }
+  size_t celem() const { return dbl() ? 16 : 8; }
+  ToepArgs targs() const {
+    ToepArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.D = D;
+    for (int d = 0; d < 3; ++d) {
+      a.N[d] = N[d];
+      a.L[d] = L[d];
+    }
+    a.P = P;
+    a.Q = Q;
+    return a;
+  }
+};
+
+extern "C" {
+
+int nfftm_toeplitz_create(nfftm_toeplitz* out, int prec, int D, const int64_t* N, void* hip_stream) {
+  g_err.clear();
+  if (!out || !N) return fail(NFFTM_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (D < 1 || D > 3) return fail(NFFTM_ERR_UNSUPPORTED, "Toeplitz embedding for D = %d (1..3 supported)", D);
+  if (prec != NFFTM_F32 && prec != NFFTM_F64) return fail(NFFTM_ERR_ARG, "precision %d", prec);
+  std::unique_ptr<nfftm_toeplitz_s> t(new nfftm_toeplitz_s());
+  t->prec = prec;
+  t->D = D;
+  t->stream = (hipStream_t)hip_stream;
+  for (int d = 0; d < D; ++d) {
+    if (N[d] < 1) return fail(NFFTM_ERR_ARG, "N[%d] = %lld", d, (long long)N[d]);
+    t->N[d] = N[d];
+    t->L[d] = 2 * N[d];
+    t->P *= N[d];
+    t->Q *= 2 * N[d];
+  }
+  HIPCHK(t->x.alloc((size_t)t->Q * t->celem()));
+  t->fft.reset(new FFTStage());
+  if (int rc = t->fft->build(D, t->L, t->dbl())) return rc;
+  *out = t.release();
+  return NFFTM_OK;
+}
+
+int nfftm_toeplitz_kernel(nfftm_toeplitz t, nfftm_plan p, void* lambda) {
+  g_err.clear();
+  if (!t || !p || !lambda) return fail(NFFTM_ERR_ARG, "NULL argument");
+  if (p->D != t->D || p->prec != t->prec) return fail(NFFTM_ERR_DIM, "plan and Toeplitz object differ in D or precision");
+  for (int d = 0; d < t->D; ++d)
+    if (p->N[d] != t->L[d])
+      return fail(NFFTM_ERR_DIM, "plan size %lld along dimension %d is not 2N = %lld", (long long)p->N[d], d + 1,
+                  (long long)t->L[d]);
+  const bool dbl = t->dbl();
+  hipStream_t s = p->stream;
+  DevBuf<char> ones;
+  HIPCHK(ones.alloc((size_t)std::max<i64>(1, p->M) * t->celem()));
+  HIPCHK(launch_fill_one(dbl, ones.p, p->M, s));
+  if (int rc = enqueue_transform(*p, false, ones.p, lambda)) return rc;  // eigMat = adjoint(p) * ones
+  if (int rc = t->fft->exec(true, lambda, t->celem(), s)) return rc;
+  HIPCHK(launch_shift_sign(dbl, lambda, t->targs(), s));  // the fftshift before the FFT
+  HIPCHK(hipStreamSynchronize(s));
+  return NFFTM_OK;
+}
+
+int nfftm_toeplitz_apply(nfftm_toeplitz t, const void* lambda, void* y) {
+  g_err.clear();
+  if (!t || !lambda || !y) return fail(NFFTM_ERR_ARG, "NULL argument");
+  const bool dbl = t->dbl();
+  const ToepArgs a = t->targs();
+  hipStream_t s = t->stream;
+  HIPCHK(launch_pad_corner(dbl, y, t->x.p, a, s));
+  if (int rc = t->fft->exec(true, t->x.p, t->celem(), s)) return rc;
+  HIPCHK(launch_cmul(dbl, t->x.p, lambda, t->Q, s));
+  if (int rc = t->fft->exec(false, t->x.p, t->celem(), s)) return rc;
+  HIPCHK(launch_crop_corner(dbl, t->x.p, y, a, 1.0 / (double)t->Q, s));
+  return NFFTM_OK;
+}
+
+void nfftm_toeplitz_destroy(nfftm_toeplitz t) {
+  if (!t) return;
+  (void)hipStreamSynchronize(t->stream);
+  delete t;
+}
+
 }  // extern "C"

@@ -317,6 +317,51 @@ class OraclePlan:
         return np.ascontiguousarray(out.T[::-1])
 
 
+def sdc(plan, iters=20, weights=None, scale=True):
+    """Sampling density compensation over the oracle's stages (FP64): sdc! of
+    NFFTTools/src/samplingDensity.jl:93-118 (Pipe & Menon iterations: spread
+    the weights, divide the grid by its first-iteration maximum, interpolate,
+    divide) and, with ``scale``, :133-155 (least-squares scale c with
+    A' diag(w) A 1 ~ 1)."""
+    w = np.ones(plan.M) if weights is None else np.array(weights, dtype=np.float64)
+    sf = None
+    for i in range(iters):
+        g = plan.convolve_transpose(w.astype(np.complex128)).real
+        if i == 0:
+            sf = g.max()
+        g = g / sf
+        wt = plan.convolve(g.astype(np.complex128)).real / sf
+        if np.any(wt <= 0):
+            raise ValueError("non-positive weights")
+        w = w / wt
+    if scale:
+        v = plan.adjoint(plan.forward(np.ones(plan.N, dtype=np.complex128)) * w)
+        w = w * (v.sum().real / np.sum(np.abs(v) ** 2))
+    return w
+
+
+def toeplitz_kernel_explicit(shape, tr):
+    """calculateToeplitzKernel_explicit of NFFTTools/src/Toeplitz.jl:133-156
+    (FP64, Python axis order): lambda = FFT(fftshift(E)) over the 2N grid,
+    E[n] = sum_j exp(2 pi i sum_d k_dj (n_d - N_d)), n_d in [0, 2 N_d) -- the
+    reference's getMatrixElement with 1-based idx = n + 1.  Formed as a sum
+    of separable products per node (chunked), then numpy's FFT."""
+    tr = np.asarray(tr, dtype=np.float64)
+    L = [2 * int(s) for s in shape]
+    D = len(L)
+    E = np.zeros(L, dtype=np.complex128)
+    for j0 in range(0, tr.shape[1], 256):
+        k = tr[:, j0:j0 + 256]
+        facs = [np.exp(2j * np.pi * np.outer(k[d], np.arange(L[d]) - L[d] // 2)) for d in range(D)]
+        if D == 1:
+            E += facs[0].sum(axis=0)
+        elif D == 2:
+            E += facs[0].T @ facs[1]
+        else:
+            E += np.einsum("ja,jb,jc->abc", facs[0], facs[1], facs[2], optimize=True)
+    return np.fft.fftn(np.fft.fftshift(E))
+
+
 def relerr(a, b):
     a = np.asarray(a).ravel()
     b = np.asarray(b).ravel()
