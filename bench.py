@@ -219,6 +219,113 @@ def run_batch(args, torch, nfftm, dev):
     print(json.dumps(out))
 
 
+def run_tools(args, torch, nfftm, dev):
+    """NFFTTools on the plan, 256^2 image on the golden-angle radial trajectory
+    (402 spokes x 512 samples): sampling density compensation (sdc, 20
+    iterations) and A'A products through the Toeplitz kernel (2x oversampled
+    FFTs) against the same products as forward + adjoint NFFT.  value = A'A
+    products per second through the Toeplitz kernel (FP32)."""
+    import nfftm.tools as T
+    k = radial_nodes(402, 512)
+    N = (256, 256)
+    M = k.shape[1]
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    res = {}
+    for name, dtype, cd in (("f32", torch.float32, torch.complex64), ("f64", torch.float64, torch.complex128)):
+        log(f"bench: tools {name}")
+        p = nfftm.plan_nfft(k, N, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=dtype)
+        w = torch.ones(M, dtype=dtype, device=dev)
+
+        def run_sdc():
+            w.fill_(1)
+            T.sdc_(p, 20, w)
+
+        t_sdc = timed(run_sdc, max(2, args.steps // 4), 1)
+        lam = T.calculateToeplitzKernel(N, k.astype(np.float32 if dtype == torch.float32 else np.float64))
+        t = T.Toeplitz(N, cd)
+        g = torch.Generator(device=dev)
+        g.manual_seed(9)
+        x = torch.randn(N, dtype=torch.complex128, device=dev, generator=g).to(cd)
+        y = x.clone()
+        ya = torch.empty(M, dtype=cd, device=dev)
+        xa = torch.empty(N, dtype=cd, device=dev)
+
+        def toep():
+            y.copy_(x)
+            t.apply(lam, y)
+
+        def nfft_pair():
+            p.forward(x, ya)
+            p.adjoint(ya, xa)
+
+        t_toep = timed(toep, args.steps, args.warmup)
+        t_pair = timed(nfft_pair, args.steps, args.warmup)
+        toep()
+        nfft_pair()
+        torch.cuda.synchronize()
+        err = float(torch.linalg.vector_norm(y - xa) / torch.linalg.vector_norm(xa))
+        res[name] = {"t_sdc": t_sdc, "t_toep": t_toep, "t_pair": t_pair, "toep_vs_nfft": err,
+                     "w": w.double().cpu().numpy()}
+        del p, lam, t
+        torch.cuda.empty_cache()
+    wf, wd = res["f32"]["w"], res["f64"]["w"]
+    Q = 4 * N[0] * N[1]
+    P = N[0] * N[1]
+    # pad (P read, Q write), FFT (Q read + write), product (2Q read, Q write),
+    # inverse FFT (Q read + write), crop (P read + write): 8 B per point
+    bytes_prod = (P + Q + 2 * Q + 3 * Q + 2 * Q + 2 * P) * 8
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        orc = O.OraclePlan(k, N, m=4, sigma=2.0, precompute=O.LINEAR)
+        t0 = time.perf_counter()
+        O.sdc(orc, iters=2, scale=False)
+        t_it = (time.perf_counter() - t0) / 2
+        cpu = {"value": 1.0 / (t_it * 20), "unit": "sdc runs/s (20 iterations)", "cores": 1, "kind": "port",
+               "sample": "2 sdc iterations of the FP64 C restatement (convolve_transpose! + convolve!) on "
+                         "the same trajectory, scaled to 20 iterations"}
+    out = {
+        "metric": "A'A products/s via the Toeplitz kernel, 256^2 radial 402x512 (FP32)",
+        "value": 1.0 / res["f32"]["t_toep"],
+        "unit": "products/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["f32"]["t_toep"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "NFFTTools: 256^2 image, golden-angle radial trajectory 402 spokes x 512 "
+                               "samples (M=205,824), m=4 sigma=2 LINEAR; sdc 20 iterations; Toeplitz "
+                               "A'A (2x oversampled FFTs) vs forward+adjoint NFFT",
+                   "N": list(N), "M": M},
+        "nfft_pair_ms": res["f32"]["t_pair"] * 1e3,
+        "toeplitz_speedup_vs_nfft_pair": res["f32"]["t_pair"] / res["f32"]["t_toep"],
+        "sdc_ms": {"f32": res["f32"]["t_sdc"] * 1e3, "f64": res["f64"]["t_sdc"] * 1e3},
+        "fp64": {"value": 1.0 / res["f64"]["t_toep"], "ms_per_step": res["f64"]["t_toep"] * 1e3},
+        "errors": {"toeplitz_vs_nfft_pair_f32": res["f32"]["toep_vs_nfft"],
+                   "toeplitz_vs_nfft_pair_f64": res["f64"]["toep_vs_nfft"],
+                   "sdc_f32_vs_f64": float(np.linalg.norm(wf - wd) / np.linalg.norm(wd))},
+        "roofline": {"bound": "hbm", "kernel": "whole Toeplitz product (pad, FFT, product, inverse FFT, crop)",
+                     "achieved": bytes_prod / res["f32"]["t_toep"] / 1e9, "peak": 8000.0, "unit": "GB/s",
+                     "frac": bytes_prod / res["f32"]["t_toep"] / 1e9 / 8000.0, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,7 +334,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--ndft-fwd", type=int, default=1024, help="sampled nodes for the NDFT check")
     ap.add_argument("--ndft-adj", type=int, default=64, help="sampled grid points for the NDFT check")
-    ap.add_argument("--workload", default="3d", choices=("3d", "1d", "2d-spiral", "batch"))
+    ap.add_argument("--workload", default="3d", choices=("3d", "1d", "2d-spiral", "batch", "tools"))
     ap.add_argument("--tune", action="store_true",
                     help="autotune the tile shape of both plans before timing (plan set-up)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -247,6 +354,8 @@ def main():
     L = nfftm._L.load()
     if args.workload == "batch":
         return run_batch(args, torch, nfftm, dev)
+    if args.workload == "tools":
+        return run_tools(args, torch, nfftm, dev)
 
     desc, N, M, m, sigma, pre_name, rng, k = workload(args.workload)
     pre = getattr(nfftm, pre_name)
