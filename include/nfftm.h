@@ -52,8 +52,18 @@ enum { NFFTM_F32 = 0, NFFTM_F64 = 1, NFFTM_MIXED = 2 /* alias of NFFTM_F32 */ };
 /* Precompute flags, same values as AbstractNFFTs/src/misc.jl:13-18. */
 enum { NFFTM_FULL = 1, NFFTM_TENSOR = 2, NFFTM_LINEAR = 3, NFFTM_POLYNOMIAL = 4 };
 
-/* Window ids (src/windowFunctions.jl:4-19).  Only Kaiser-Bessel is built. */
-enum { NFFTM_KAISER_BESSEL = 0 };
+/* Window ids (src/windowFunctions.jl:4-19, getWindow): :kaiser_bessel,
+ * :gauss, :spline, :kaiser_bessel_rev, :cosh_type (pairs psi, psi_hat at
+ * :21-134).  The node gradient with precompute LINEAR or FULL is built for
+ * Kaiser-Bessel only (the analytic window derivative); TENSOR and POLYNOMIAL
+ * differentiate the fitted polynomial of any window. */
+enum {
+  NFFTM_KAISER_BESSEL = 0,
+  NFFTM_GAUSS = 1,
+  NFFTM_SPLINE = 2,
+  NFFTM_KAISER_BESSEL_REV = 3,
+  NFFTM_COSH_TYPE = 4
+};
 
 /* Plan flags. */
 enum {

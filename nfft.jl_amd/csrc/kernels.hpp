@@ -95,6 +95,7 @@ struct KArgs {
   const void* poly; // R[(2m+1) * 2m], column l holds coefficients of tap l
   const double* polyd;  // the same coefficients in FP64 (node gradient)
   double b;         // Kaiser-Bessel shape pi*(2 - 1/sigma)
+  int win;          // window id (NFFTM_KAISER_BESSEL .. NFFTM_COSH_TYPE) of the exact (FULL) mode
   double wscale;    // power-of-two window normalisation
   double oscale;    // output scale: 1 inside a transform; wscale^-D for the
                     // standalone convolve!/convolve_transpose! entry points
@@ -127,6 +128,62 @@ __device__ __forceinline__ double scaled_node(double ksv, double Ntd) {
 // Window values of one node along dimension d.  Returns the wrapped cell
 // floor(k * Ntilde) (in [0, Ntilde)); t = kscale - off with
 // off = cell - m + 1 exactly as src/precomputation.jl:537-545.
+// I0 by its positive power series, to the precision of R.
+template <typename R>
+__device__ __forceinline__ R i0_dev(R x) {
+  const R y = (R)0.25 * x * x;
+  R t = 1, s = 1;
+  for (int k = 1; k < 200 && t > (sizeof(R) == 4 ? (R)1e-8 : (R)1e-17) * s; ++k) {
+    t = t * y / (R)(k * k);
+    s += t;
+  }
+  return s;
+}
+
+// Exact window psi(u), u in grid units, of window a.win (src/windowFunctions.jl:
+// 21-134): Kaiser-Bessel, Gaussian, cardinal B-spline of order W = 2m
+// (Cox-de Boor), reversed Kaiser-Bessel, cosh type.
+template <typename R, int W>
+__device__ __forceinline__ R window_exact(const KArgs& a, R u) {
+  const R m = (R)a.m, b = (R)a.b, au = u < 0 ? -u : u;
+  switch (a.win) {
+    case 1: {  // gauss, b_g = m / pi
+      const R bg = m / (R)M_PI;
+      return au < m ? exp(-u * u / bg) / sqrt((R)M_PI * bg) : (R)0;
+    }
+    case 2: {  // spline: M_W(u + m)
+      if (!(au < m)) return 0;
+      const R x = u + m;
+      R v[W];
+#pragma unroll
+      for (int s = 0; s < W; ++s) v[s] = (x - (R)s >= 0 && x - (R)s < 1) ? (R)1 : (R)0;
+#pragma unroll
+      for (int k = 2; k <= W; ++k) {
+#pragma unroll
+        for (int s = 0; s + k <= W; ++s) v[s] = ((x - (R)s) * v[s] + ((R)k - (x - (R)s)) * v[s + 1]) / (R)(k - 1);
+      }
+      return v[0];
+    }
+    case 3: {  // kaiser_bessel_rev
+      if (!(au < m)) return 0;
+      const R r = u / m;
+      return (R)0.5 / m * i0_dev<R>(m * b * sqrt((R)1 - r * r));
+    }
+    case 4: {  // cosh_type, beta = m b
+      if (!(au < m)) return 0;
+      const R r = u / m, al = sqrt((R)1 - r * r), beta = m * b;
+      return (cosh(beta * al) - (R)1) / ((cosh(beta) - (R)1) * al);
+    }
+    default: {  // kaiser_bessel
+      if (au < m) {
+        const R arg = sqrt(m * m - u * u);
+        return sinh(b * arg) / (arg * (R)M_PI);
+      }
+      return au > m ? (R)0 : b / (R)M_PI;
+    }
+  }
+}
+
 template <typename R, int W>
 __device__ __forceinline__ int node_weights(const KArgs& a, int d, double ksv, R* w) {
   const double kscale = scaled_node(ksv, a.Ntd[d]);
@@ -158,22 +215,8 @@ __device__ __forceinline__ int node_weights(const KArgs& a, int d, double ksv, R
       w[l] = s;
     }
   } else {
-    const R m = (R)a.m, b = (R)a.b;
 #pragma unroll
-    for (int l = 0; l < W; ++l) {
-      R u = (R)(t - (double)l);
-      R au = u < 0 ? -u : u;
-      R y;
-      if (au < m) {
-        R arg = sqrt(m * m - u * u);
-        y = sinh(b * arg) / (arg * (R)M_PI);
-      } else if (au > m) {
-        y = 0;
-      } else {
-        y = b / (R)M_PI;
-      }
-      w[l] = y * (R)a.wscale;
-    }
+    for (int l = 0; l < W; ++l) w[l] = window_exact<R, W>(a, (R)(t - (double)l)) * (R)a.wscale;
   }
   return (int)(c >= a.Nt[d] ? c - a.Nt[d] : c);
 }

@@ -73,6 +73,89 @@ double kb_psi_hat(double n, double Nt, int m, double b) {
   return i0_series(m * std::sqrt(b * b - t * t));
 }
 
+// ---------------------------------------------------------------------------
+// The other windows of src/windowFunctions.jl:41-134 (ids as the C ABI's
+// NFFTM_GAUSS .. NFFTM_COSH_TYPE), psi in grid units u = Ntilde k.
+// ---------------------------------------------------------------------------
+// J0(x) = (1/pi) int_0^pi cos(x sin t) dt by the trapezoidal rule on the
+// periodic integrand, which converges geometrically: 48 + |x| nodes give full
+// FP64 precision for the arguments of the cosh-type transform.
+double j0_trapz(double x) {
+  const int n = 48 + (int)std::ceil(std::fabs(x));
+  double s = 0.5 * (1.0 + std::cos(0.0));  // t = 0 and t = pi ends: cos(0) each, halved
+  for (int i = 1; i < n; ++i) s += std::cos(x * std::sin(M_PI * i / n));
+  return s / n;
+}
+
+// Cardinal B-spline of order n at x (cbspline, src/windowFunctions.jl:65-77),
+// Cox-de Boor over the shifts.
+double bspline(int n, double x) {
+  std::vector<double> v(n);
+  for (int s = 0; s < n; ++s) v[s] = (x - s >= 0.0 && x - s < 1.0) ? 1.0 : 0.0;
+  for (int k = 2; k <= n; ++k)
+    for (int s = 0; s + k <= n; ++s) v[s] = ((x - s) * v[s] + (k - (x - s)) * v[s + 1]) / (k - 1);
+  return v[0];
+}
+
+double win_psi(int win, double u, int m, double sigma) {
+  const double au = std::fabs(u);
+  switch (win) {
+    case NFFTM_KAISER_BESSEL:
+      return kb_psi(u, m, M_PI * (2.0 - 1.0 / sigma));
+    case NFFTM_GAUSS: {
+      const double b = m / M_PI;
+      return au < m ? std::exp(-u * u / b) / std::sqrt(M_PI * b) : 0.0;
+    }
+    case NFFTM_SPLINE:
+      return au < m ? bspline(2 * m, u + m) : 0.0;
+    case NFFTM_KAISER_BESSEL_REV: {
+      const double b = M_PI * (2.0 - 1.0 / sigma), r = u / m;
+      return au < m ? 0.5 / m * i0_series(m * b * std::sqrt(1.0 - r * r)) : 0.0;
+    }
+    case NFFTM_COSH_TYPE: {
+      const double beta = M_PI * m * (2.0 - 1.0 / sigma), r = u / m;
+      if (!(au < m)) return 0.0;
+      const double al = std::sqrt(1.0 - r * r);
+      return (std::cosh(beta * al) - 1.0) / ((std::cosh(beta) - 1.0) * al);
+    }
+  }
+  return 0.0;
+}
+
+double win_psi_hat(int win, double n, double Nt, int m, double sigma) {
+  switch (win) {
+    case NFFTM_KAISER_BESSEL:
+      return kb_psi_hat(n, Nt, m, M_PI * (2.0 - 1.0 / sigma));
+    case NFFTM_GAUSS: {
+      const double t = M_PI * n / Nt;
+      return std::exp(-t * t * (m / M_PI));
+    }
+    case NFFTM_SPLINE: {
+      const double x = n / Nt;
+      return std::pow(x == 0.0 ? 1.0 : std::sin(M_PI * x) / (M_PI * x), 2 * m);
+    }
+    case NFFTM_KAISER_BESSEL_REV: {
+      // real(sinc(sqrt(complex(q)) / pi)): sin(a)/a for q >= 0, sinh(y)/y below
+      const double b = M_PI * (2.0 - 1.0 / sigma), t = 2.0 * M_PI * m * n / Nt;
+      const double q = t * t - (m * b) * (m * b);
+      if (q >= 0.0) {
+        const double a = std::sqrt(q);
+        return a == 0.0 ? 1.0 : std::sin(a) / a;
+      }
+      const double y = std::sqrt(-q);
+      return std::sinh(y) / y;
+    }
+    case NFFTM_COSH_TYPE: {
+      const double beta = M_PI * m * (2.0 - 1.0 / sigma), gam = beta / (2.0 * M_PI);
+      const double zeta = M_PI * m / (std::cosh(beta) - 1.0), arg = m * n / Nt, w = 2.0 * M_PI * arg;
+      if (std::fabs(arg) < gam) return zeta * (i0_series(std::sqrt(beta * beta - w * w)) - j0_trapz(w));
+      if (std::fabs(arg) > gam) return zeta * (j0_trapz(std::sqrt(w * w - beta * beta)) - j0_trapz(w));
+      return zeta * (1.0 - j0_trapz(beta));
+    }
+  }
+  return 1.0;
+}
+
 // Chebyshev interpolant with n first-kind nodes on [a, b]
 // (BasicInterpolators.ChebyshevInterpolator, used at src/precomputation.jl:351):
 // coefficients from the discrete cosine sums, evaluation by Clenshaw.
@@ -111,7 +194,7 @@ i64 lut_size(int m) {
 // (src/precomputation.jl:302-320): sample points s_q = -1/2 + q/(2*deg-1),
 // q = 0..2*deg-1, deg = 2m+1 monomials; column l (0-based) fits
 // psi(m - (l + 1/2) + x).  Solved by Householder QR of the Vandermonde matrix.
-std::vector<double> poly_fit(int m, double b) {
+std::vector<double> poly_fit(int win, int m, double sigma) {
   const int deg = 2 * m + 1, W = 2 * m, ns = 2 * deg;
   std::vector<double> x(ns);
   for (int q = 0; q < ns; ++q) x[q] = (double)(2 * q - (ns - 1)) / (double)(2 * (ns - 1));
@@ -150,7 +233,7 @@ std::vector<double> poly_fit(int m, double b) {
   for (int l = 0; l < W; ++l) {
     std::vector<double> y(ns);
     const double c0 = (double)m - ((double)l + 0.5);
-    for (int q = 0; q < ns; ++q) y[q] = kb_psi(c0 + x[q], m, b);
+    for (int q = 0; q < ns; ++q) y[q] = win_psi(win, c0 + x[q], m, sigma);
     for (int c = 0; c < deg; ++c) {
       double s = 0;
       for (int q = c; q < ns; ++q) s += house[c][q] * y[q];
@@ -447,6 +530,7 @@ struct nfftm_plan_s {
     a.poly = d_poly.p;
     a.polyd = d_polyd.p;
     a.b = b;
+    a.win = window;
     a.wscale = wscale;
     a.oscale = 1.0;
     a.ks = d_ks.p;
@@ -634,8 +718,8 @@ bool tiles_ok(const nfftm_plan_s& p) {
 // Window normalisation by an exact power of two so the FP32 weights stay near
 // 1 (the unnormalised psi(0) reaches 1e15 at m = 8); the deconvolution factors
 // carry the inverse power, so the transform is unchanged.
-double window_scale(int m, double b) {
-  const double p0 = kb_psi(0.0, m, b);
+double window_scale(int win, int m, double sigma) {
+  const double p0 = win_psi(win, 0.0, m, sigma);
   return std::ldexp(1.0, -(int)std::lround(std::log2(p0)));
 }
 
@@ -742,21 +826,22 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
 int build_tables(nfftm_plan_s& p) {
   const int m = p.m;
   p.b = M_PI * (2.0 - 1.0 / p.sigma);
-  p.wscale = window_scale(m, p.b);
+  p.wscale = window_scale(p.window, m, p.sigma);
   // bound of one window value: psi(0) for the table and the exact window; the
   // fitted polynomials stay within a hair of psi (5 % slack)
-  p.wmax = kb_psi(0.0, m, p.b) * p.wscale * (p.precompute == NFFTM_LINEAR || p.precompute == NFFTM_FULL ? 1.0 : 1.05);
+  p.wmax = win_psi(p.window, 0.0, m, p.sigma) * p.wscale *
+           (p.precompute == NFFTM_LINEAR || p.precompute == NFFTM_FULL ? 1.0 : 1.05);
   hipError_t e = hipSuccess;
   if (p.precompute == NFFTM_LINEAR) {
     // precomputeLinInterp (src/precomputation.jl:291-300): K+2 samples, step m/K
     const i64 K = p.LUTSize;
     p.lut.resize(K + 2);
     const double step = (double)m / (double)K;
-    for (i64 l = 0; l < K + 2; ++l) p.lut[l] = kb_psi((double)l * step, m, p.b);
+    for (i64 l = 0; l < K + 2; ++l) p.lut[l] = win_psi(p.window, (double)l * step, m, p.sigma);
     if (p.dbl()) upload_table<double>(p.d_lut, p.lut, p.wscale, e);
     else upload_table<float>(p.d_lut, p.lut, p.wscale, e);
   } else if (p.precompute == NFFTM_TENSOR || p.precompute == NFFTM_POLYNOMIAL) {
-    p.poly = poly_fit(m, p.b);
+    p.poly = poly_fit(p.window, m, p.sigma);
     if (p.dbl()) upload_table<double>(p.d_poly, p.poly, p.wscale, e);
     else upload_table<float>(p.d_poly, p.poly, p.wscale, e);
     if (e == hipSuccess) {
@@ -779,7 +864,7 @@ int build_tables(nfftm_plan_s& p) {
     p.deconv[d].resize(p.N[d]);
     const i64 Na = p.N[d] / 2;
     const double Ntd = (double)p.Nt[d];
-    auto kappa = [&](double j) { return kb_psi_hat(j - 1.0 - (double)Na, Ntd, m, p.b); };
+    auto kappa = [&](double j) { return win_psi_hat(p.window, j - 1.0 - (double)Na, Ntd, m, p.sigma); };
     if (p.N[d] > 1) {
       ChebInterp cheb(kappa, 1.0, (double)p.N[d], 30);
       for (i64 i = 0; i < p.N[d]; ++i) p.deconv[d][i] = 1.0 / cheb((double)(i + 1)) / p.wscale;
@@ -1358,7 +1443,8 @@ int nfftm_plan_create(nfftm_plan* out, int prec, int D, const int64_t* N, int64_
   for (int d = 0; d < D; ++d)
     if (N[d] < 1) return fail(NFFTM_ERR_ARG, "N[%d] = %lld must be positive", d, (long long)N[d]);
   if (M > 0 && !k_host) return fail(NFFTM_ERR_ARG, "k is NULL");
-  if (window != NFFTM_KAISER_BESSEL) return fail(NFFTM_ERR_UNSUPPORTED, "Window %d not yet implemented!", window);
+  if (window < NFFTM_KAISER_BESSEL || window > NFFTM_COSH_TYPE)
+    return fail(NFFTM_ERR_UNSUPPORTED, "Window %d not yet implemented!", window);
   if (precompute < NFFTM_FULL || precompute > NFFTM_POLYNOMIAL)
     return fail(NFFTM_ERR_UNSUPPORTED, "precompute = %d not supported", precompute);
   if (m < 1 || m > 8) return fail(NFFTM_ERR_UNSUPPORTED, "m = %d not supported (1..8)", m);
@@ -1489,6 +1575,10 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
 int nfftm_grad_nodes(nfftm_plan p, const void* f, const void* ybar, double* kbar) {
   g_err.clear();
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if (p->window != NFFTM_KAISER_BESSEL && (p->precompute == NFFTM_LINEAR || p->precompute == NFFTM_FULL))
+    return fail(NFFTM_ERR_UNSUPPORTED,
+                "node gradient of window %d needs precompute TENSOR or POLYNOMIAL (the analytic window "
+                "derivative is built for Kaiser-Bessel)", p->window);
   if ((!f && p->P) || (!ybar && p->M) || (!kbar && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
   if (int rc = forward_to_grid(*p, f, [] { return (int)NFFTM_OK; })) return rc;
   if (p->M) {

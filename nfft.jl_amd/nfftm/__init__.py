@@ -44,7 +44,7 @@ __all__ = [
 
 # PrecomputeFlags, AbstractNFFTs/src/misc.jl:13-18
 FULL, TENSOR, LINEAR, POLYNOMIAL = _L.FULL, _L.TENSOR, _L.LINEAR, _L.POLYNOMIAL
-_WINDOWS = {"kaiser_bessel": _L.KAISER_BESSEL}
+_WINDOWS = {"kaiser_bessel": 0, "gauss": 1, "spline": 2, "kaiser_bessel_rev": 3, "cosh_type": 4}
 
 
 class ArgumentError(ValueError):
@@ -239,7 +239,7 @@ class NFFTPlan:
         if k.shape[0] != D:
             raise ArgumentError(f"Nodes x have dimension {k.shape[0]} != {D}")
         m, sigma, reltol = accuracy_params(**kw)
-        window = kw.get("window", "kaiser_bessel")
+        window = str(kw.get("window", "kaiser_bessel")).lstrip(":")  # a Julia symbol name works too
         if window not in _WINDOWS:
             raise NFFTError(f"Window {window} not yet implemented!")
         precompute = int(kw.get("precompute", POLYNOMIAL))

@@ -70,7 +70,13 @@ def lib():
         L.orc_deconv_factors.argtypes = [LL, LL, I, D, I, dp]
         L.orc_num_threads.restype = I
         L.orc_plan_create.restype = P
-        L.orc_plan_create.argtypes = [I, lp, LL, dp, I, D, I, I, I]
+        L.orc_plan_create.argtypes = [I, lp, LL, dp, I, D, I, I, I, I]
+        L.orc_window_w.restype = D
+        L.orc_window_w.argtypes = [I, D, D, I, D]
+        L.orc_window_hat_w.restype = D
+        L.orc_window_hat_w.argtypes = [I, D, D, I, D]
+        L.orc_bessel_j0.restype = D
+        L.orc_bessel_j0.argtypes = [D]
         L.orc_plan_destroy.argtypes = [P]
         L.orc_plan_info.argtypes = [P, lp, lp, dp]
         for name in ("orc_deconvolve", "orc_deconvolve_transpose", "orc_convolve",
@@ -222,17 +228,36 @@ def window_du(u, m, sigma):
 # ----------------------------------------------------------------------------
 # The plan (src/implementation.jl)
 # ----------------------------------------------------------------------------
+# window ids of src/windowFunctions.jl:4-19 (getWindow)
+WINDOWS = {"kaiser_bessel": 0, "gauss": 1, "spline": 2, "kaiser_bessel_rev": 3, "cosh_type": 4}
+
+
+def window_w(window, k, Nt, m, sigma):
+    """psi(k) of a named window (src/windowFunctions.jl:21-134)."""
+    return lib().orc_window_w(WINDOWS[window], float(k), float(Nt), int(m), float(sigma))
+
+
+def window_hat_w(window, n, Nt, m, sigma):
+    """psi_hat(n) of a named window (src/windowFunctions.jl:21-134)."""
+    return lib().orc_window_hat_w(WINDOWS[window], float(n), float(Nt), int(m), float(sigma))
+
+
+def bessel_j0(x):
+    return lib().orc_bessel_j0(float(x))
+
+
 class OraclePlan:
     """FP64 CPU restatement of ``NFFTPlan`` (src/implementation.jl:16-193)."""
 
     def __init__(self, k, N, m=4, sigma=2.0, precompute=LINEAR, blocking=True, cheb=True,
-                 workers=1):
+                 workers=1, window="kaiser_bessel"):
         kj, D, M = _julia_nodes(k)
         self.D, self.M, self.N = D, M, tuple(int(n) for n in N)
         assert len(self.N) == D
         Na = np.array(self.N[::-1], dtype=np.int64)
         self._p = lib().orc_plan_create(D, _lp(Na), M, _dp(kj), int(m), float(sigma),
-                                        int(precompute), int(bool(blocking)), int(bool(cheb)))
+                                        int(precompute), int(bool(blocking)), int(bool(cheb)),
+                                        WINDOWS[window])
         if not self._p:
             raise ValueError("nodes must lie in [-1/2, 1/2]")
         Nt = np.zeros(D, dtype=np.int64)

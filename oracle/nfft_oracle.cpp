@@ -30,10 +30,9 @@ namespace {
 constexpr int MAXD = 8;
 
 // --------------------------------------------------------------------------
-// Window functions: src/windowFunctions.jl:4-19 (getWindow), :21-34 (KB),
-// :36-39 (KB hat).  Window ids: 0 kaiser_bessel, 1 gauss, 2 spline,
-// 3 kaiser_bessel_rev, 4 cosh_type.  Only Kaiser-Bessel is restated with
-// its transform; others return NaN (out of scope of this checker).
+// Window functions: src/windowFunctions.jl:4-19 (getWindow) and the pairs
+// (psi, psi_hat) at :21-134.  Window ids: 0 kaiser_bessel, 1 gauss,
+// 2 spline, 3 kaiser_bessel_rev, 4 cosh_type.
 // --------------------------------------------------------------------------
 
 // Modified Bessel function I0 by its power series sum_k (x^2/4)^k / (k!)^2.
@@ -70,6 +69,93 @@ double window_kb_hat(double n, double Nt, int m, double sigma) {
   double b = M_PI * (2.0 - 1.0 / sigma);
   double t = 2.0 * M_PI * n / Nt;
   return bessel_i0(m * std::sqrt(b * b - t * t));
+}
+
+// Bessel function J0 by its power series sum_k (-x^2/4)^k / (k!)^2 in long
+// double (64-bit mantissa): the alternating terms lose ~log10 of the largest
+// term, < 5 digits for the arguments of the cosh-type transform (|x| <= 13).
+double bessel_j0(double x) {
+  long double q = -0.25L * (long double)x * (long double)x, term = 1.0L, s = 1.0L;
+  for (int k = 1; k < 400; ++k) {
+    term *= q / ((long double)k * (long double)k);
+    s += term;
+    if (k > std::fabs(x) && std::fabs((double)term) < 1e-24 * std::fabs((double)s)) break;
+  }
+  return (double)s;
+}
+
+// Cardinal B-spline of order n at x (src/windowFunctions.jl:65-77 cbspline):
+// the same recursion, tabulated over the shifts it visits.
+double cbspline(int n, double x) {
+  std::vector<double> v(n);
+  for (int s = 0; s < n; ++s) v[s] = (x - s >= 0.0 && x - s < 1.0) ? 1.0 : 0.0;
+  for (int k = 2; k <= n; ++k)
+    for (int s = 0; s + k <= n; ++s)
+      v[s] = (x - s) / (k - 1) * v[s] + (k - (x - s)) / (k - 1) * v[s + 1];
+  return v[0];
+}
+
+inline double jl_sinc(double x) { return x == 0.0 ? 1.0 : std::sin(M_PI * x) / (M_PI * x); }
+
+// psi(k) of window `win` (k in units of the period, support |k| < m / Nt).
+double window_psi(int win, double k, double Nt, int m, double sigma) {
+  const double ak = std::fabs(k), mN = m / Nt;
+  switch (win) {
+    case 0:
+      return window_kb(k, Nt, m, sigma);
+    case 1: {  // :54-62
+      const double b = m / M_PI;
+      return ak < mN ? 1.0 / std::sqrt(M_PI * b) * std::exp(-(Nt * k) * (Nt * k) / b) : 0.0;
+    }
+    case 2:  // :79-86
+      return ak < mN ? cbspline(2 * m, Nt * k + m) : 0.0;
+    case 3: {  // :41-49
+      const double b = M_PI * (2.0 - 1.0 / sigma);
+      if (!(ak < mN)) return 0.0;
+      const double r = Nt * k / m;
+      return 0.5 / m * bessel_i0(m * b * std::sqrt(1.0 - r * r));
+    }
+    case 4: {  // :95-107
+      const double beta = M_PI * m * (2.0 - 1.0 / sigma);
+      if (!(ak < mN)) return 0.0;
+      const double arg = Nt * k / m, alpha = std::sqrt(1.0 - arg * arg);
+      return 1.0 / (std::cosh(beta) - 1.0) * (std::cosh(beta * alpha) - 1.0) / alpha;
+    }
+  }
+  return std::nan("");
+}
+
+// psi_hat(n) of window `win`.
+double window_hat(int win, double n, double Nt, int m, double sigma) {
+  switch (win) {
+    case 0:
+      return window_kb_hat(n, Nt, m, sigma);
+    case 1: {  // :64-67
+      const double b = m / M_PI, t = M_PI * n / Nt;
+      return std::exp(-t * t * b);
+    }
+    case 2:  // :88-90
+      return std::pow(jl_sinc(n / Nt), 2 * m);
+    case 3: {  // :51-56: real(sinc(sqrt(complex(x)) / pi))
+      const double b = M_PI * (2.0 - 1.0 / sigma);
+      const double t = 2.0 * M_PI * m * n / Nt, x = t * t - (m * b) * (m * b);
+      if (x >= 0.0) {
+        const double a = std::sqrt(x);
+        return a == 0.0 ? 1.0 : std::sin(a) / a;
+      }
+      const double y = std::sqrt(-x);
+      return std::sinh(y) / y;
+    }
+    case 4: {  // :109-127
+      const double beta = M_PI * m * (2.0 - 1.0 / sigma), gam = beta / (2.0 * M_PI);
+      const double zeta = M_PI / (std::cosh(beta) - 1.0) * m, arg = m * n / Nt;
+      const double w = 2.0 * M_PI * arg;
+      if (std::fabs(arg) < gam) return zeta * (bessel_i0(std::sqrt(beta * beta - w * w)) - bessel_j0(w));
+      if (std::fabs(arg) > gam) return zeta * (bessel_j0(std::sqrt(w * w - beta * beta)) - bessel_j0(w));
+      return zeta * (1.0 - bessel_j0(beta));
+    }
+  }
+  return std::nan("");
 }
 
 // Derivative d psi / du of the Kaiser-Bessel window of src/windowFunctions.jl:
@@ -121,9 +207,9 @@ i64 block_size(int D, const i64* Nt, int d) {
 
 // src/precomputation.jl:291-300 (precomputeLinInterp): K+2 samples of the
 // positive half of the window, step m/K, evaluated with Ntilde = 1.
-void lin_interp_table(int m, double sigma, i64 K, double* out) {
+void lin_interp_table(int win, int m, double sigma, i64 K, double* out) {
   double step = (double)m / (double)K;
-  for (i64 l = 0; l < K + 2; ++l) out[l] = window_kb((double)l * step, 1.0, m, sigma);
+  for (i64 l = 0; l < K + 2; ++l) out[l] = window_psi(win, (double)l * step, 1.0, m, sigma);
 }
 
 // Least squares min ||A c - y||_2 by Householder QR (A is rows x cols,
@@ -161,7 +247,7 @@ void lstsq(int rows, int cols, std::vector<double> A, std::vector<double> y, dou
 // src/precomputation.jl:302-320 (precomputePolyInterp).  Output P is
 // (2m+1) x 2m column-major: column l holds the monomial coefficients
 // (degree 0..2m) of the fit of psi(m - (l - 1/2) + x), x in [-1/2, 1/2].
-void poly_interp_table(int m, double sigma, double* P) {
+void poly_interp_table(int win, int m, double sigma, double* P) {
   int deg = 2 * m + 1, K = 2 * m, NS = 2 * deg;
   std::vector<double> k(NS);
   for (int s = 0; s < NS; ++s)  // range(-0.5, 0.5, length=NS), correctly rounded
@@ -178,7 +264,7 @@ void poly_interp_table(int m, double sigma, double* P) {
   for (int l = 1; l <= K; ++l) {
     std::vector<double> y(NS);
     double c0 = -((double)l - 0.5) + m;
-    for (int s = 0; s < NS; ++s) y[s] = window_kb(c0 + k[s], 1.0, m, sigma);
+    for (int s = 0; s < NS; ++s) y[s] = window_psi(win, c0 + k[s], 1.0, m, sigma);
     lstsq(NS, deg, A, y, P + (l - 1) * deg);
   }
 }
@@ -221,9 +307,9 @@ struct Cheb {
 // src/precomputation.jl:347-358 (precomputeWindowHatInvLUT), one dimension.
 // cheb != 0: the reference's 30-node Chebyshev interpolant; cheb == 0: the
 // exact 1/psi_hat(n).
-void deconv_factors(i64 N, i64 Nt, int m, double sigma, int cheb, double* out) {
+void deconv_factors(int win, i64 N, i64 Nt, int m, double sigma, int cheb, double* out) {
   i64 off = index_offset(N);
-  auto kappa = [&](double n) { return window_kb_hat(n + (double)off, (double)Nt, m, sigma); };
+  auto kappa = [&](double n) { return window_hat(win, n + (double)off, (double)Nt, m, sigma); };
   if (cheb && N > 1) {
     Cheb ch(kappa, 1.0, (double)N, 30);
     for (i64 j = 1; j <= N; ++j) out[j - 1] = 1.0 / ch((double)j);
@@ -247,7 +333,7 @@ inline double frac_prod(double k, double n) {
 // The plan: src/implementation.jl:16-43 (NFFTPlan), :73-106 (constructor).
 // ==========================================================================
 struct OPlan {
-  int D, m, precompute, blocking;
+  int D, m, precompute, blocking, win = 0;
   i64 N[MAXD], Nt[MAXD], M, P, Q, LUTSize;
   double sigma;
   std::vector<double> k;  // D x M, unshifted (the plan's k)
@@ -370,7 +456,7 @@ void node_weights_nonblocked(const OPlan& p, i64 j, int d, i64* idx, double* w) 
   for (int l = 0; l < Z; ++l) idx[l] = (((i64)l + off + Nt) % Nt + Nt) % Nt;
   if (p.precompute == FULL) {  // :124-134
     for (int l = 0; l < Z; ++l)
-      w[l] = window_kb((kscale - (double)l - (double)off) / (double)Nt, (double)Nt, m, p.sigma);
+      w[l] = window_psi(p.win, (kscale - (double)l - (double)off) / (double)Nt, (double)Nt, m, p.sigma);
   } else if (p.precompute == LINEAR) {  // :138-150
     double idxf = ((kscale - (double)off) * (double)p.LUTSize) / (double)m;
     lin_weights(p, idxf, p.LUTSize / m, w);
@@ -424,7 +510,7 @@ void node_weights_grad(const OPlan& p, i64 j, int d, i64* idx, double* w, double
     for (int l = 0; l < Z; ++l) {
       double u = kscale - (double)l - (double)off;
       w[l] = window_kb(u / (double)Nt, (double)Nt, m, p.sigma);
-      dw[l] = (double)Nt * window_kb_du(u, m, p.sigma);
+      dw[l] = p.win == 0 ? (double)Nt * window_kb_du(u, m, p.sigma) : std::nan("");
     }
   }
 }
@@ -653,11 +739,16 @@ extern "C" {
 double orc_bessel_i0(double x) { return bessel_i0(x); }
 double orc_window(double k, double Nt, int m, double sigma) { return window_kb(k, Nt, m, sigma); }
 double orc_window_hat(double n, double Nt, int m, double sigma) { return window_kb_hat(n, Nt, m, sigma); }
+double orc_window_w(int win, double k, double Nt, int m, double sigma) { return window_psi(win, k, Nt, m, sigma); }
+double orc_window_hat_w(int win, double n, double Nt, int m, double sigma) {
+  return window_hat(win, n, Nt, m, sigma);
+}
+double orc_bessel_j0(double x) { return bessel_j0(x); }
 long long orc_lut_size(int m) { return lut_size(m); }
-void orc_lut(int m, double sigma, long long K, double* out) { lin_interp_table(m, sigma, K, out); }
-void orc_poly(int m, double sigma, double* out) { poly_interp_table(m, sigma, out); }
+void orc_lut(int m, double sigma, long long K, double* out) { lin_interp_table(0, m, sigma, K, out); }
+void orc_poly(int m, double sigma, double* out) { poly_interp_table(0, m, sigma, out); }
 void orc_deconv_factors(long long N, long long Nt, int m, double sigma, int cheb, double* out) {
-  deconv_factors(N, Nt, m, sigma, cheb, out);
+  deconv_factors(0, N, Nt, m, sigma, cheb, out);
 }
 int orc_num_threads() {
 #ifdef _OPENMP
@@ -670,7 +761,8 @@ int orc_num_threads() {
 // src/implementation.jl:73-106 with initParams (src/precomputation.jl:3-56).
 // Returns NULL on invalid nodes (src/utils.jl:46-55).
 OPlan* orc_plan_create(int D, const long long* N, long long M, const double* k, int m, double sigma,
-                       int precompute, int blocking, int cheb_deconv) {
+                       int precompute, int blocking, int cheb_deconv, int window) {
+  if (window < 0 || window > 4) return nullptr;
   if (D < 1 || D > MAXD) return nullptr;
   for (long long j = 0; j < (long long)D * M; ++j)
     if (!(std::fabs(k[j]) <= 0.5)) return nullptr;
@@ -679,6 +771,7 @@ OPlan* orc_plan_create(int D, const long long* N, long long M, const double* k, 
   p->m = m;
   p->precompute = precompute;
   p->blocking = blocking;
+  p->win = window;
   p->M = M;
   p->P = 1;
   p->Q = 1;
@@ -693,16 +786,16 @@ OPlan* orc_plan_create(int D, const long long* N, long long M, const double* k, 
   p->k.assign(k, k + (size_t)D * M);
   if (precompute == LINEAR) {
     p->lut.resize(p->LUTSize + 2);
-    lin_interp_table(m, p->sigma, p->LUTSize, p->lut.data());
+    lin_interp_table(p->win, m, p->sigma, p->LUTSize, p->lut.data());
   }
   if (precompute == TENSOR || precompute == POLYNOMIAL) {
     p->poly.resize((2 * m + 1) * 2 * m);
-    poly_interp_table(m, p->sigma, p->poly.data());
+    poly_interp_table(p->win, m, p->sigma, p->poly.data());
   }
   p->deconv.resize(D);
   for (int d = 0; d < D; ++d) {
     p->deconv[d].resize(p->N[d]);
-    deconv_factors(p->N[d], p->Nt[d], m, p->sigma, cheb_deconv, p->deconv[d].data());
+    deconv_factors(p->win, p->N[d], p->Nt[d], m, p->sigma, cheb_deconv, p->deconv[d].data());
   }
   // calcBlocks: src/implementation.jl:182-187
   if (blocking && precompute != FULL) precompute_blocks(*p);

@@ -68,7 +68,7 @@ def test_check_nodes_is_argument_error(k, msg):
 
 def test_unsupported_options():
     k = np.zeros((2, 3))
-    assert _create(k, (4, 4), window=3)[0] == _lib.ERR_UNSUPPORTED
+    assert _create(k, (4, 4), window=7)[0] == _lib.ERR_UNSUPPORTED
     assert _create(k, (4, 4), precompute=7)[0] == _lib.ERR_UNSUPPORTED
     assert _create(k, (4, 4), m=9)[0] == _lib.ERR_UNSUPPORTED
     assert _create(k, (4, 4), prec=5)[0] == _lib.ERR_UNSUPPORTED
