@@ -65,8 +65,10 @@ end
 precision_of(::Type{Float32}) = NFFTM_F32
 precision_of(::Type{Float64}) = NFFTM_F64
 
-window_id(w::Symbol) = w == :kaiser_bessel ? Cint(0) :
-  error("Window $w is not available in the MI355X backend")
+# getWindow's names (src/windowFunctions.jl:4-19) and the C ABI's ids
+const WINDOW_IDS = Dict(:kaiser_bessel => Cint(0), :gauss => Cint(1), :spline => Cint(2),
+                        :kaiser_bessel_rev => Cint(3), :cosh_type => Cint(4))
+window_id(w::Symbol) = haskey(WINDOW_IDS, w) ? WINDOW_IDS[w] : error("Window $(w) not yet implemented!")
 
 function wrap_grid(h::Ptr{Cvoid}, ::Type{T}, Ñ::NTuple{D,Int}) where {T,D}
   ptr = Ref{Ptr{Cvoid}}()
