@@ -618,7 +618,124 @@ def plan_nfft(*args, **kw):
     if active_backend() is None:
         raise NFFTError("No default backend available!")
     k, N = args[0], args[1]
+    dims = kw.pop("dims", None)
+    Nt = (int(N),) if np.isscalar(N) else tuple(int(n) for n in N)
+    if dims is not None and tuple(np.atleast_1d(dims)) != tuple(range(len(Nt))):
+        return DirectionalPlan(k, Nt, dims, **kw)
     return NFFTPlan(k, N, **kw)
+
+
+class DirectionalPlan:
+    """NFFT along a contiguous range of axes of a larger array
+    (``plan_nfft(k, N; dims)``, src/directional.jl:1-156 with initParams'
+    dims handling at src/precomputation.jl:3-29).  The other axes are batch
+    axes: the transform equals the ``len(dims)``-dimensional NFFT of every
+    slice, and the output replaces the transformed axes by one axis of
+    length J.  ``dims`` are axes of the C-order array (an int or a range of
+    consecutive ints); node row d pairs with axis dims[d].
+
+    The slices are transformed by one plan over N[dims]: as one batch of
+    slices sharing its nodes (nfftm_batch_*) when len(dims) <= 2, slice by
+    slice otherwise.  Moving the batch axes aside is a device permutation."""
+
+    def __init__(self, k, N, dims, **kw):
+        d = np.atleast_1d(np.asarray(list(dims) if isinstance(dims, range) else dims, dtype=np.int64))
+        D = len(N)
+        if d.ndim != 1 or len(d) == 0 or np.any(np.diff(d) != 1) or d[0] < 0 or d[-1] >= D:
+            raise ArgumentError(f"dims must be a range of consecutive axes of a {D}-dimensional array")
+        k = _as_nodes(k)
+        if k.shape[0] != len(d):
+            raise ArgumentError(f"Nodes x have dimension {k.shape[0]} != {len(d)}")
+        self.N = tuple(N)
+        self.dims = tuple(int(x) for x in d)
+        self.D = D
+        self.sub = NFFTPlan(k, tuple(N[x] for x in self.dims), **kw)
+        self.J = self.sub.J
+        self.dtype = self.sub.dtype
+        self._pre = self.N[:self.dims[0]]
+        self._post = self.N[self.dims[-1] + 1:]
+        self.B = int(np.prod(self._pre + self._post, dtype=np.int64))
+        self.NOut = self._pre + (self.J,) + self._post
+        self._batch = self._make_batch()
+
+    def _make_batch(self):
+        # plans the batch kernels do not take (D = 3, node-parallel tiny
+        # grids) run slice by slice
+        if self.sub.D > 2 or self.B < 2:
+            return None
+        try:
+            return BatchPlan([self.sub], [self.B], autotune=False)
+        except NFFTError:
+            return None
+
+    # the batch axes first, then the transformed axes (image) or J (nodes)
+    def _to_slices(self, x, inner):
+        n0, n1 = len(self._pre), len(self._pre) + len(inner)
+        perm = list(range(n0)) + list(range(n1, x.dim())) + list(range(n0, n1))
+        return x.permute(perm).reshape((self.B,) + tuple(inner)).contiguous()
+
+    def _from_slices(self, y, inner):
+        y = y.reshape(self._pre + self._post + tuple(inner))
+        n0, nb = len(self._pre), len(self._pre) + len(self._post)
+        perm = list(range(n0)) + list(range(nb, nb + len(inner))) + list(range(n0, nb))
+        return y.permute(perm).contiguous()
+
+    def _run(self, x, shape_in, inner_in, inner_out, fwd):
+        x = _to_dev(x, self.dtype)
+        if tuple(x.shape) != tuple(shape_in):
+            raise DimensionMismatch(f"Data is not consistent with the directional plan (shape "
+                                    f"{tuple(x.shape)}, expected {tuple(shape_in)})")
+        xs = self._to_slices(x, inner_in)
+        ys = torch.empty((self.B,) + tuple(inner_out), dtype=self.dtype, device=_device())
+        if self._batch is not None:
+            fn = self._batch.forward if fwd else self._batch.adjoint
+            fn([xs[b] for b in range(self.B)], [ys[b] for b in range(self.B)])
+        else:
+            for b in range(self.B):
+                if fwd:
+                    self.sub.forward(xs[b], ys[b])
+                else:
+                    self.sub.adjoint(xs[b], ys[b])
+        return self._from_slices(ys, inner_out)
+
+    def forward(self, f, out=None):
+        """mul!(fHat, p, f) for a directional plan."""
+        y = self._run(f, self.N, self.sub.N, (self.J,), True)
+        if out is not None:
+            out.copy_(y)
+            return out
+        return y
+
+    def adjoint(self, fhat, out=None):
+        """mul!(f, adjoint(p), fHat) for a directional plan."""
+        x = self._run(fhat, self.NOut, (self.J,), self.sub.N, False)
+        if out is not None:
+            out.copy_(x)
+            return out
+        return x
+
+    def size_in(self):
+        return self.N
+
+    def size_out(self):
+        return self.NOut
+
+    @property
+    def H(self):
+        return AdjointPlan(self)
+
+    def __mul__(self, f):
+        return self.forward(f)
+
+    def nodes_(self, k):
+        """nodes!(p, k): new nodes for every slice (same J)."""
+        self.sub.nodes_(k)
+        self._batch = self._make_batch()
+        return self
+
+    def __repr__(self):
+        return (f"NFFTPlan with {self.J} sampling points for an input array of size{self.N} and an "
+                f"output array of size{self.NOut} with dims {self.dims}")
 
 
 def mul_(out, p, inp, **kw):

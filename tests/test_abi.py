@@ -98,9 +98,13 @@ def test_host_mirror_errors_map_to_reference_classes():
     with pytest.raises(nfftm.ArgumentError):
         nfftm.plan_nfft(np.array([[-0.6, 0.9], [0.5, -0.5]]), (2, 2))
     with pytest.raises(nfftm.NFFTError):
-        nfftm.plan_nfft(np.zeros((2, 4)), (2, 2), window="gauss")
-    with pytest.raises(nfftm.NFFTError):
+        nfftm.plan_nfft(np.zeros((2, 4)), (2, 2), window="hann")
+    # directional plans: node rows must match len(dims) (src/precomputation.jl:19-21),
+    # dims a range of consecutive axes
+    with pytest.raises(nfftm.ArgumentError):
         nfftm.plan_nfft(np.zeros((2, 4)), (2, 2), dims=1)
+    with pytest.raises(nfftm.ArgumentError):
+        nfftm.plan_nfft(np.zeros((2, 4)), (2, 2, 2), dims=(0, 2))
 
 
 def test_accuracy_params_rules():
