@@ -20,14 +20,18 @@
 // zero-padded grid (FFTW's forward / unnormalised backward transform,
 // src/implementation.jl:87-88) up to rounding.
 //
-// One workgroup transforms E / L lines of length L = 2^k in LDS (FP32:
-// E = 4096 points, 256 threads, 16 <= L <= 2048; FP64: E = 2048, 128
-// threads, 16 <= L <= 1024): a first radix-2^(k mod 4) pass, then radix-16
-// Stockham passes, L / 16 threads per line, one radix-16 butterfly (or
-// 16 / R radix-R ones) per thread and pass, twiddles from a table of
-// exp(-2 pi i j / L) in LDS.  Strided passes (Y, Z) take E / L lines that
-// are consecutive along x, so that every row of the load and store is a
-// contiguous run of E / L elements.
+// One workgroup transforms E / L lines of length L = 2^k (FP32: E = 4096
+// points, 256 threads, 16 <= L <= 2048; FP64: E = 2048, 128 threads,
+// 16 <= L <= 1024): a first radix-2^(k mod 4) pass (radix 16 when k is a
+// multiple of 4), then radix-16 Stockham passes, L / 16 threads per line,
+// one radix-16 butterfly (or 16 / R radix-R ones) per thread and pass,
+// twiddles from a table of exp(-2 pi i j / L) in LDS.  A thread loads the 16
+// points of its first-pass butterflies and stores the 16 outputs of its
+// last-pass butterfly, so the first and last passes run on registers and
+// only the passes between them exchange through LDS (for L = 256: one LDS
+// write, one barrier, one read).  Strided passes (Y, Z) take E / L lines
+// that are consecutive along x, so that every row of the load and store is
+// a contiguous run of E / L elements.
 #include "launch.hpp"
 
 namespace nfftm {
@@ -140,25 +144,55 @@ __device__ __forceinline__ void stockham(C* line, int j0, const C* tw) {
   __syncthreads();
 }
 
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+// Radix of the first pass: 2^(log2 L mod 4), or 16 when L is a power of 16.
+template <int L>
+constexpr int first_radix() {
+  return (ilog2(L) % 4) ? (1 << (ilog2(L) % 4)) : 16;
+}
+
+// Radix-16 Stockham passes of stages NS, 16 NS, ... below L / 16, in LDS.
 template <typename R, int L, int NS, bool INV, typename C>
-__device__ __forceinline__ void passes16(C* line, int j0, const C* tw) {
-  if constexpr (NS < L) {
+__device__ __forceinline__ void mid_passes(C* line, int j0, const C* tw) {
+  if constexpr (NS < L / 16) {
     stockham<R, L, 16, NS, INV>(line, j0, tw);
-    passes16<R, L, NS * 16, INV>(line, j0, tw);
+    mid_passes<R, L, NS * 16, INV>(line, j0, tw);
   }
 }
 
-constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
-
-// FFT of the line (all threads of the workgroup call this together).
+// FFT of a line of L points.  On entry the thread holds x[u] = point
+// j0 + u L / 16 of its line, on exit output point j0 + u L / 16: these are
+// exactly the inputs of the first pass's butterflies j0 + b L / 16 (stage 1,
+// radix R0, b < 16 / R0) and the outputs of the last pass's butterfly j0
+// (stage L / 16, radix 16), so both passes run on registers and only the
+// passes between them go through the LDS line.  All threads of the
+// workgroup call this together; which line and j0 a thread takes may differ
+// from the (line, j0) = (t / (L / 16), t mod (L / 16)) of the middle passes.
 template <typename R, int L, bool INV, typename C>
-__device__ __forceinline__ void line_fft(C* line, int j0, const C* tw) {
-  constexpr int R0 = 1 << (ilog2(L) % 4);
-  if constexpr (R0 > 1) {
-    stockham<R, L, R0, 1, INV>(line, j0, tw);
-    passes16<R, L, R0, INV>(line, j0, tw);
+__device__ __forceinline__ void line_fft(C (&x)[16], C* line, int j0, C* mline, int mj0, const C* tw) {
+  constexpr int TL = L / 16;
+  if constexpr (L == 16) {
+    dft<R, 16, INV>(x);
   } else {
-    passes16<R, L, 1, INV>(line, j0, tw);
+    constexpr int R0 = first_radix<L>(), B = 16 / R0;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      C v[R0];
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[r] = x[b + r * B];  // point j0 + b TL + r L / R0
+      dft<R, R0, INV>(v);
+      const int o = (j0 + b * TL) * R0;
+#pragma unroll
+      for (int r = 0; r < R0; ++r) line[o + r] = v[r];
+    }
+    __syncthreads();
+    mid_passes<R, L, R0, INV>(mline, mj0, tw);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] = line[j0 + r * TL];
+#pragma unroll
+    for (int r = 1; r < 16; ++r) x[r] = tmul<INV>(x[r], tw[j0 * r]);
+    dft<R, 16, INV>(x);
   }
 }
 
@@ -211,16 +245,17 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_fwd(FArgs a, cons
   const int l = threadIdx.x / G::TL, j0 = threadIdx.x % G::TL;
   const int q = blockIdx.x * G::LPB + l;
   C* line = buf + l * G::LS;
+  C x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = C{0, 0};
   if (q < nlines) {
     const int iy = q % N1, iz = q / N1;
     const double fr = line_factor(a, iy, iz, true);
     const C* src = f + (size_t)q * N0;
-    C x[16];
     double fc[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int ix = image_idx(j0 + k * G::TL, N0, L, Na0);
-      x[k] = C{0, 0};
       fc[k] = 0;
       if (ix >= 0) {
         x[k] = src[ix];
@@ -230,23 +265,16 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_fwd(FArgs a, cons
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const double s = fc[k] * fr;
-      line[j0 + k * G::TL] = C{(R)((double)x[k].x * s), (R)((double)x[k].y * s)};
+      x[k] = C{(R)((double)x[k].x * s), (R)((double)x[k].y * s)};
     }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) line[j0 + k * G::TL] = C{0, 0};
   }
-  __syncthreads();
-  line_fft<R, L, false>(line, j0, tw);
+  line_fft<R, L, false>(x, line, j0, line, j0, tw);
   if (q < nlines) {
     const int iy = q % N1, iz = q / N1;
     const size_t o = (size_t)grid_pos(iz, a.N[2], a.L[2], a.Na[2]) * a.gstr[2] +
                      (size_t)grid_pos(iy, N1, a.L[1], a.Na[1]) * a.gstr[1];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int p = j0 + k * G::TL;
-      grid[o + p] = line[p];
-    }
+    for (int k = 0; k < 16; ++k) grid[o + j0 + k * G::TL] = x[k];
   }
 }
 
@@ -266,29 +294,28 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_adj(FArgs a, cons
   const int l = threadIdx.x / G::TL, j0 = threadIdx.x % G::TL;
   const int q = blockIdx.x * G::LPB + l;
   C* line = buf + l * G::LS;
+  C x[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = C{0, 0};
   if (q < nlines) {
     const int iy = q % N1, iz = q / N1;
     const size_t o = (size_t)grid_pos(iz, a.N[2], a.L[2], a.Na[2]) * a.gstr[2] +
                      (size_t)grid_pos(iy, N1, a.L[1], a.Na[1]) * a.gstr[1];
-    C x[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[k] = grid[o + j0 + k * G::TL];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) line[j0 + k * G::TL] = x[k];
-  } else {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) line[j0 + k * G::TL] = C{0, 0};
   }
-  __syncthreads();
-  line_fft<R, L, true>(line, j0, tw);
+  line_fft<R, L, true>(x, line, j0, line, j0, tw);
   if (q < nlines) {
     const int iy = q % N1, iz = q / N1;
-    const double fr = line_factor(a, iy, iz, false);
+    const double fr = line_factor(a, iy, iz, false) * a.oscale;
     C* dst = f + (size_t)q * N0;
-    for (int ix = j0; ix < N0; ix += G::TL) {
-      const C v = line[grid_pos(ix, N0, L, Na0)];
-      const double s = a.fac[0][ix] * fr * a.oscale;
-      dst[ix] = C{(R)((double)v.x * s), (R)((double)v.y * s)};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int ix = image_idx(j0 + k * G::TL, N0, L, Na0);
+      if (ix >= 0) {
+        const double s = a.fac[0][ix] * fr;
+        dst[ix] = C{(R)((double)x[k].x * s), (R)((double)x[k].y * s)};
+      }
     }
   }
 }
@@ -296,7 +323,9 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_adj(FArgs a, cons
 // Strided pass along dimension d (1 or 2) over lines (x, c): x in
 // [0, Nt_0) in chunks of LPB, c the position along the remaining dimension
 // o (all of [0, Nt_o) if call, else S_o).  MODE 1: input nonzero on S_d
-// only (forward); MODE 2: output kept on S_d only (adjoint).
+// only (forward); MODE 2: output kept on S_d only (adjoint).  Thread t
+// takes line x0 + t mod LPB and points t / LPB + u L / 16, so that a
+// wavefront's loads and stores are rows of LPB consecutive x.
 template <typename R, int L, bool INV, int MODE>
 __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_col(FArgs a, typename cpx<R>::T* __restrict__ grid, int d, int call) {
   using C = typename cpx<R>::T;
@@ -309,59 +338,23 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_col(FArgs a, typena
   const int c = blockIdx.y;
   const int gc = call ? c : grid_pos(c, a.N[o], a.L[o], a.Na[o]);
   const int x0 = blockIdx.x * G::LPB;
-  const int L0 = a.L[0];
   const size_t base = (size_t)gc * a.gstr[o] + x0;
   const long long st = a.gstr[d];
   const int Nd = a.N[d], Nad = a.Na[d];
-  // every load of a thread is issued before its LDS writes
-  constexpr int U = G::E / G::NT;
-  C v[U];
-  if (MODE == 1) {
-    // S_d positions from the grid, the gap [Nd - Nad, L - Nad) zero
-    const int cnt = G::LPB * Nd;
+  const int xl = threadIdx.x % G::LPB, j0 = threadIdx.x / G::LPB;
+  const bool xin = x0 + xl < a.L[0];
+  C v[16];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = threadIdx.x + u * G::NT;
-      const int xl = e % G::LPB, i = e / G::LPB;
-      if (e < cnt) v[u] = x0 + xl < L0 ? grid[base + xl + (size_t)grid_pos(i, Nd, L, Nad) * st] : C{0, 0};
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = threadIdx.x + u * G::NT;
-      const int xl = e % G::LPB, i = e / G::LPB;
-      if (e < cnt) buf[xl * G::LS + grid_pos(i, Nd, L, Nad)] = v[u];
-    }
-    for (int e = threadIdx.x; e < G::LPB * (L - Nd); e += G::NT) {
-      const int xl = e % G::LPB, k = Nd - Nad + e / G::LPB;
-      buf[xl * G::LS + k] = C{0, 0};
-    }
-  } else {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = threadIdx.x + u * G::NT;
-      const int xl = e % G::LPB, k = e / G::LPB;
-      v[u] = x0 + xl < L0 ? grid[base + xl + (size_t)k * st] : C{0, 0};
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = threadIdx.x + u * G::NT;
-      buf[(e % G::LPB) * G::LS + e / G::LPB] = v[u];
-    }
+  for (int u = 0; u < 16; ++u) {
+    const int k = j0 + u * G::TL;
+    v[u] = C{0, 0};
+    if (xin && (MODE == 2 || image_idx(k, Nd, L, Nad) >= 0)) v[u] = grid[base + xl + (size_t)k * st];
   }
-  __syncthreads();
-  const int l = threadIdx.x / G::TL, j0 = threadIdx.x % G::TL;
-  line_fft<R, L, INV>(buf + l * G::LS, j0, tw);
-  if (MODE == 2) {
-    for (int e = threadIdx.x; e < G::LPB * Nd; e += G::NT) {
-      const int xl = e % G::LPB, i = e / G::LPB;
-      const int k = grid_pos(i, Nd, L, Nad);
-      if (x0 + xl < L0) grid[base + xl + (size_t)k * st] = buf[xl * G::LS + k];
-    }
-  } else {
-    for (int e = threadIdx.x; e < G::LPB * L; e += G::NT) {
-      const int xl = e % G::LPB, k = e / G::LPB;
-      if (x0 + xl < L0) grid[base + xl + (size_t)k * st] = buf[xl * G::LS + k];
-    }
+  line_fft<R, L, INV>(v, buf + xl * G::LS, j0, buf + (threadIdx.x / G::TL) * G::LS, threadIdx.x % G::TL, tw);
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int k = j0 + u * G::TL;
+    if (xin && (MODE == 1 || image_idx(k, Nd, L, Nad) >= 0)) grid[base + xl + (size_t)k * st] = v[u];
   }
 }
 
