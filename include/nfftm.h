@@ -71,7 +71,7 @@ enum {
   NFFTM_FLAG_TIMING = 2,   /* record per-stage device times (TimingStats) */
   NFFTM_FLAG_GRAPHS = 4,   /* replay repeated (input, output) transforms from HIP
                               graphs captured at the pair's second call */
-  NFFTM_FLAG_ROCFFT = 8    /* FFT stage as rocFFT over the whole oversampled grid
+  NFFTM_FLAG_ROCFFT = 8,   /* FFT stage as rocFFT over the whole oversampled grid
                               with separate deconvolution passes.  Default on
                               power-of-two grids (FP32: 16..2048 points per
                               dimension, FP64: 16..1024, D <= 3): pruned line
@@ -81,6 +81,12 @@ enum {
                               the grid (nfftm_grid) then holds the full FFT
                               after a forward transform and a partial one
                               after an adjoint. */
+  NFFTM_FLAG_SEPARATE_FOLD = 16 /* keep the 3-D adjoint's fold and its first
+                              pruned FFT pass as two kernels.  By default they
+                              are one kernel where every tile holds at most
+                              one work item and the tile shape matches the
+                              pass (e.g. 256-point z lines): bit-identical
+                              results, the folded grid is never written. */
 };
 
 /* Plan creation: replaces NFFTPlan(k, N; m, sigma, window, precompute,

@@ -5,6 +5,7 @@
 // for D = 3 so that the LDS strides are constants (immediate ds offsets, no
 // integer divisions), with 1024-thread workgroups for latency hiding and two
 // nodes in flight per wavefront.
+#include "fft_line.hpp"
 #include "launch.hpp"
 
 namespace nfftm {
@@ -547,6 +548,128 @@ __global__ __launch_bounds__(256) void k_fold3b(KArgs a, typename cpx<R>::T* __r
 #pragma unroll
     for (int k = 0; k < TZ; ++k) grid[((long long)(z0 + k) * Ny + y) * Nx + x] = acc[k];
   }
+}
+
+// Adjoint with the fold fused into the first pruned FFT pass (lines along
+// dimension 3, inverse FFT, output kept on S_z), for plans whose tiles hold
+// at most one work item each.  Workgroup (ox, y) takes the T lines x of tile
+// column ox at row y (E / L = T lines of L points); thread (lx, j0) sums its
+// 16 points z = j0 + u L / 16 from the slots exactly as k_fold3b sums them
+// (same neighbours in the same order, so the grid values are bit-identical),
+// transforms them with the thread mapping of k_fft_col and stores the S_z
+// positions.  A wavefront's lanes share the tile row of each point
+// ((64 / T) | TZ), so tile and item lookups are scalar.
+template <typename R, int W, int V, int L>
+struct FoldFft3 {
+  using G = Geo3<R, W, V>;
+  using LG = lf::LineGeo<R, L>;
+  static constexpr bool OK = G::VALID && LG::LPB == G::T && G::T <= 64 && 64 % G::T == 0 &&
+                             LG::TL % G::TZ == 0 && G::TZ % (64 / G::T) == 0;
+};
+
+template <typename R, int W, int V, int L>
+__global__ __launch_bounds__((lf::LineGeo<R, L>::NT)) void k_fold_fft3(KArgs a, FArgs f,
+                                                                      typename cpx<R>::T* __restrict__ grid) {
+  using C = typename cpx<R>::T;
+  using G = Geo3<R, W, V>;
+  using B = Blocks3<W, G::T, G::TZ>;
+  using LG = lf::LineGeo<R, L>;
+  constexpr int T = G::T, TZ = G::TZ, m = W / 2, TL = LG::TL;
+  constexpr int NZ = L / TZ;  // tile rows along z
+  __shared__ __align__(16) R sm[2 * (LG::LPB * LG::LS + L)];
+  __shared__ int item[9][NZ];  // work item of tile (ox + dx, oy + dy, z row), -1 if none
+  C* buf = (C*)sm;
+  C* tw = buf + LG::LPB * LG::LS;
+  lf::load_tw<LG::NT>(tw, f.tw[2], L);
+  const C* scr = (const C*)a.scratch;
+  const int ntx = a.ntiles[0], nty = a.ntiles[1];
+  const int ox = blockIdx.x, y = blockIdx.y, oy = y / T, ly = y % T;
+  for (int i = threadIdx.x; i < 9 * NZ; i += LG::NT) {
+    const int dxy = i / NZ, tz = i % NZ;
+    const int tx = (ox + dxy % 3 - 1 + ntx) % ntx, ty = (oy + dxy / 3 - 1 + nty) % nty;
+    const int t = tx + ntx * (ty + nty * tz);
+    const int i0 = a.tstart[t];
+    item[dxy][tz] = a.tstart[t + 1] > i0 ? i0 : -1;
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % T, j0 = threadIdx.x / T, lz = j0 % TZ;
+  // point u lies in tile row q + u TL / TZ
+  const int q = __builtin_amdgcn_readfirstlane(j0 / TZ);
+  C v[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) v[u] = C{0, 0};
+  static_for<0, 27>([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    constexpr int dx = d % 3 - 1, dy = (d / 3) % 3 - 1, dz = d / 9 - 1;
+    constexpr int bx = 1 - dx, by = 1 - dy, bz = 1 - dz;
+    constexpr int sx = dx == 1 ? T - m + 1 : 0, sy = dy == 1 ? T - m + 1 : 0, sz = dz == 1 ? TZ - m + 1 : 0;
+    constexpr int zlo = dz == 1 ? TZ - m + 1 : 0, zhi = dz == -1 ? m : TZ;
+    constexpr int ex = B::ext(bx), ey = B::ext(by);
+    constexpr int off = B::off(bx + 3 * by + 9 * bz);
+    if (dy == 0 || (dy == -1 ? ly < m : ly >= T - m + 1)) {
+      const bool use = (dx == 0 || (dx == -1 ? lx < m : lx >= T - m + 1)) && lz >= zlo && lz < zhi;
+      if (use) {
+        const int* row = item[(dy + 1) * 3 + dx + 1];
+        const unsigned li = off + (lx - sx) + ex * (ly - sy) + ex * ey * (lz - sz);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int it = row[(q + u * (TL / TZ) + dz + NZ) % NZ];
+          const C w = (scr + (size_t)(it >= 0 ? it : 0) * G::SLOT)[li];
+          v[u].x += it >= 0 ? w.x : (R)0;
+          v[u].y += it >= 0 ? w.y : (R)0;
+        }
+      }
+    }
+  });
+  lf::line_fft<R, L, true>(v, buf + lx * LG::LS, j0, buf + (threadIdx.x / TL) * LG::LS, threadIdx.x % TL, tw);
+  const size_t base = (size_t)y * f.gstr[1] + (size_t)ox * T;
+  const long long st = f.gstr[2];
+  const int Nd = f.N[2], Nad = f.Na[2];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int k = j0 + u * TL;
+    if (lf::image_idx(k, Nd, L, Nad) >= 0) grid[base + lx + (size_t)k * st] = v[u];
+  }
+}
+
+template <typename R, int W, int V, int L>
+static hipError_t fold_fft3(const KArgs& a, const FArgs& f, hipStream_t s, void* grid, bool dry) {
+  if constexpr (!FoldFft3<R, W, V, L>::OK) {
+    return hipErrorNotSupported;
+  } else {
+    if (dry) return hipSuccess;
+    hipLaunchKernelGGL((k_fold_fft3<R, W, V, L>), dim3((unsigned)a.ntiles[0], (unsigned)f.L[1]),
+                       dim3(lf::LineGeo<R, L>::NT), 0, s, a, f, (typename cpx<R>::T*)grid);
+    return hipGetLastError();
+  }
+}
+
+template <typename R, int W, int V>
+static hipError_t fold_fft3_l(const KArgs& a, const FArgs& f, hipStream_t s, void* grid, bool dry) {
+  switch (f.L[2]) {
+    case 128: return fold_fft3<R, W, V, 128>(a, f, s, grid, dry);
+    case 256: return fold_fft3<R, W, V, 256>(a, f, s, grid, dry);
+    case 512: return fold_fft3<R, W, V, 512>(a, f, s, grid, dry);
+    default: return hipErrorNotSupported;
+  }
+}
+
+template <typename R, int W>
+static hipError_t fold_fft3_v(const KArgs& a, const FArgs& f, hipStream_t s, void* grid, bool dry) {
+  return a.tvar ? fold_fft3_l<R, W, 1>(a, f, s, grid, dry) : fold_fft3_l<R, W, 0>(a, f, s, grid, dry);
+}
+
+hipError_t launch_fold_fft3(bool dbl, int W, const KArgs& a, const FArgs& f, hipStream_t s, void* grid, bool dry) {
+  if (!a.blocked3 || f.D != 3) return hipErrorNotSupported;
+#define NFFTM_FF(WW) \
+  case WW:           \
+    return dbl ? fold_fft3_v<double, WW>(a, f, s, grid, dry) : fold_fft3_v<float, WW>(a, f, s, grid, dry);
+  switch (W) {
+    NFFTM_FF(2) NFFTM_FF(4) NFFTM_FF(6) NFFTM_FF(8) NFFTM_FF(10) NFFTM_FF(12) NFFTM_FF(14) NFFTM_FF(16)
+    default:
+      return hipErrorNotSupported;
+  }
+#undef NFFTM_FF
 }
 
 template <typename R, int W, int V>

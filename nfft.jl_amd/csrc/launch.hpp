@@ -37,6 +37,11 @@ struct FArgs {
 // Power-of-two grids within the line kernels' LDS (FP32: 16..2048 points
 // per dimension, FP64: 16..1024), D <= 3, N_d <= Nt_d.
 bool pruned_fft_supported(int D, const int64_t* N, const int64_t* Nt, bool dbl);
+// Adjoint, 3-D block-layout slots with at most one work item per tile: the
+// fold fused into the first pruned pass (dimension 3), grid positions S_z of
+// every (x, y) line written.  hipErrorNotSupported (nothing launched) where
+// the tile geometry does not match the pass; dry = only that check.
+hipError_t launch_fold_fft3(bool dbl, int W, const KArgs& a, const FArgs& f, hipStream_t s, void* grid, bool dry);
 hipError_t launch_pruned_fft_fwd(bool dbl, const FArgs& a, hipStream_t s, const void* f, void* grid, int part);
 hipError_t launch_pruned_fft_adj(bool dbl, const FArgs& a, hipStream_t s, void* grid, void* f, int part);
 

@@ -412,6 +412,7 @@ struct nfftm_plan_s {
   DevBuf<int> d_tstart;
   DevBuf<char> d_scratch, d_rec;
   int nitems = 0;
+  bool single_items = false;  // tiled, nitems > 0, no tile with two work items
   int gen = 0;        // incremented whenever the node arrays are rebuilt
   double wmax = 1.0;  // bound of one (normalised) window value
 
@@ -803,6 +804,8 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
     tstart[ntile] = (int)items.size();
   }
   p.nitems = (int)items.size();
+  p.single_items = p.tiled && p.nitems > 0;
+  for (size_t t = 0; p.single_items && t + 1 < tstart.size(); ++t) p.single_items = tstart[t + 1] - tstart[t] <= 1;
   HIPCHK(p.d_ks.alloc(ks.size()));
   HIPCHK(p.d_perm.alloc(perm.size()));
   HIPCHK(p.d_items.alloc(items.size()));
@@ -891,7 +894,14 @@ struct ProfScope {
   }
 };
 
-int run_conv(nfftm_plan_s& p, bool spread, const void* in, void* out, bool standalone = false) {
+// The 3-D adjoint's fold runs inside its first pruned FFT pass (see
+// launch_fold_fft3) unless NFFTM_FLAG_SEPARATE_FOLD is set.
+bool fused_fold(const nfftm_plan_s& p) {
+  return p.pfft && p.blocked3() && p.single_items && !(p.flags & NFFTM_FLAG_SEPARATE_FOLD) &&
+         launch_fold_fft3(p.dbl(), p.W, p.kargs(), p.fargs(), nullptr, nullptr, true) == hipSuccess;
+}
+
+int run_conv(nfftm_plan_s& p, bool spread, const void* in, void* out, bool standalone = false, bool fold = true) {
   KArgs a = p.kargs();
   if (standalone) a.oscale = std::pow(p.wscale, -(double)p.D);
   hipError_t e;
@@ -907,7 +917,7 @@ int run_conv(nfftm_plan_s& p, bool spread, const void* in, void* out, bool stand
       e = launch_generic(spread, p.dbl(), p.W, a, p.stream, in, out);
   }
   HIPCHK(e);
-  if (spread && p.tiled) {
+  if (spread && p.tiled && fold) {
     ProfScope ps(p, 3);
     HIPCHK(launch_fold(p.D, p.dbl(), p.W, a, p.stream, out));
   }
@@ -946,14 +956,19 @@ int forward_to_grid(nfftm_plan_s& p, const void* f, Mark&& mark) {
 }
 
 // Adjoint inverse FFT of the grid and deconvolution into f: stage 5 and
-// stage 6, mark in between.
+// stage 6, mark in between.  fused: the scratch slots are not folded yet and
+// the first pruned pass folds them (stage 3).
 template <typename Mark>
-int grid_to_image(nfftm_plan_s& p, void* f, Mark&& mark) {
+int grid_to_image(nfftm_plan_s& p, void* f, Mark&& mark, bool fused = false) {
   if (p.pfft) {
     const FArgs a = p.fargs();
+    if (fused) {
+      ProfScope ps(p, 3);
+      HIPCHK(launch_fold_fft3(p.dbl(), p.W, p.kargs(), a, p.stream, p.d_grid.p, false));
+    }
     {
       ProfScope ps(p, 5);
-      HIPCHK(launch_pruned_fft_adj(p.dbl(), a, p.stream, p.d_grid.p, f, 0));
+      HIPCHK(launch_pruned_fft_adj(p.dbl(), a, p.stream, p.d_grid.p, f, fused ? 2 : 0));
     }
     if (int rc = mark()) return rc;
     ProfScope ps(p, 6);
@@ -1492,8 +1507,9 @@ int enqueue_transform(nfftm_plan_s& p, bool fwd, const void* in, void* out) {
     if (p.M) return run_conv(p, false, p.d_grid.p, out);
     return NFFTM_OK;
   }
-  if (int rc = run_conv(p, true, in, p.d_grid.p)) return rc;
-  return grid_to_image(p, out, none);
+  const bool fused = fused_fold(p);
+  if (int rc = run_conv(p, true, in, p.d_grid.p, false, !fused)) return rc;
+  return grid_to_image(p, out, none, fused);
 }
 
 // Graph replay of enqueue_transform for a repeated (input, output) pair.
@@ -1565,9 +1581,10 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
     return graph_transform(*p, false, fhat, f);
   Timer t(*p);
   if (int rc = t.mark(0)) return rc;
-  if (int rc = run_conv(*p, true, fhat, p->d_grid.p)) return rc;
+  const bool fused = fused_fold(*p);
+  if (int rc = run_conv(*p, true, fhat, p->d_grid.p, false, !fused)) return rc;
   if (int rc = t.mark(1)) return rc;
-  if (int rc = grid_to_image(*p, f, [&] { return t.mark(2); })) return rc;
+  if (int rc = grid_to_image(*p, f, [&] { return t.mark(2); }, fused)) return rc;
   if (int rc = t.mark(3)) return rc;
   return t.finish(&p->timing[4], &p->timing[5], &p->timing[6]);
 }

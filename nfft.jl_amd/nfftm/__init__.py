@@ -256,7 +256,8 @@ class NFFTPlan:
         kj = np.ascontiguousarray(k[::-1].T, dtype=np.float64)
         Nj = (ctypes.c_int64 * D)(*N[::-1])
         flags = ((_L.FLAG_BLOCKING if blocking else 0) | (_L.FLAG_GRAPHS if kw.get("graphs", False) else 0)
-                 | (_L.FLAG_ROCFFT if kw.get("rocfft", False) else 0))
+                 | (_L.FLAG_ROCFFT if kw.get("rocfft", False) else 0)
+                 | (0 if kw.get("fused_fold", True) else _L.FLAG_SEPARATE_FOLD))
         _check(self._lib.nfftm_plan_create(
             ctypes.byref(self._handle), prec, D, Nj, k.shape[1],
             kj.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m, sigma, _WINDOWS[window],

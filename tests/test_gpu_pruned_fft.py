@@ -142,3 +142,60 @@ def test_pruned_graph_replay():
         assert rel(y, y0) < 1e-6, it
         assert rel(x, x0) < 1e-6, it
 
+
+
+# 3-D adjoint with the fold fused into the first pruned pass (z lines of 256
+# points; FP32 16 x 16 x 8 tiles, FP64 8 x 8 x 8): N in C order, z first
+FUSED = [((128, 16, 16), torch.float32), ((128, 32, 16), torch.float32),
+         ((128, 16, 16), torch.float64), ((128, 16, 32), torch.float64)]
+
+
+@pytest.mark.parametrize("N,dtype", FUSED, ids=[f"{'x'.join(map(str, c[0]))}-{str(c[1])[6:]}" for c in FUSED])
+def test_fused_fold_first_pass(N, dtype):
+    """The fused adjoint equals the fold + separate pass (bit for bit in FP32,
+    whose spreading sums in fixed point; FP64 spreading adds floats in LDS in
+    arrival order, so two runs agree to rounding: <= 1e-14), never writes the
+    folded grid (z positions outside S_z keep what the grid held), and
+    matches the oracle (FP64 <= 1e-12, FP32 <= 1e-5 against the FP64
+    oracle)."""
+    print(f"fused fold N={N} {dtype}", flush=True)
+    rng = np.random.default_rng([sum(N), dtype == torch.float64])
+    M = 20000
+    k = rng.random((3, M)) - 0.5
+    kw = dict(m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=dtype)
+    p = nfftm.plan_nfft(k, N, **kw)
+    q = nfftm.plan_nfft(k, N, fused_fold=False, **kw)
+    cd = torch.complex128 if dtype == torch.float64 else torch.complex64
+    fh = torch.as_tensor(crandn(rng, M), dtype=cd, device="cuda")
+    gap = slice(N[0] // 2, 2 * N[0] - N[0] // 2)  # z positions outside S_z
+    for plan in (p, q):
+        plan.tmpVec.fill_(7.0)
+    same = torch.equal if dtype == torch.float32 else (lambda a, b: rel(a, b) < 1e-14)
+    x, xq = p.H * fh, q.H * fh
+    torch.cuda.synchronize()
+    assert same(x, xq), rel(x, xq)
+    assert bool((p.tmpVec[gap] == 7.0).all())
+    assert not bool((q.tmpVec[gap] == 7.0).all())
+    # a second adjoint on new values through the same plan
+    fh2 = torch.as_tensor(crandn(rng, M), dtype=cd, device="cuda")
+    assert same(p.H * fh2, q.H * fh2)
+    orc = O.OraclePlan(k, N, m=4, sigma=2.0, precompute=O.LINEAR)
+    assert rel(x, orc.adjoint(fh.cpu().numpy())) < (1e-12 if dtype == torch.float64 else 1e-5)
+
+
+def test_fused_fold_falls_back_on_crowded_tiles():
+    """A tile with more nodes than one work item holds keeps the separate fold
+    (the folded grid is written) and still matches the oracle."""
+    print("fused fold fallback", flush=True)
+    rng = np.random.default_rng(33)
+    N = (128, 16, 16)
+    k = rng.random((3, 12000)) - 0.5
+    k[:, :3000] = 0.01 * rng.random((3, 3000))  # 3000 nodes in one tile
+    p = nfftm.plan_nfft(k, N, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=torch.float64)
+    fh = torch.as_tensor(crandn(rng, k.shape[1]), dtype=torch.complex128, device="cuda")
+    p.tmpVec.fill_(7.0)
+    x = p.H * fh
+    torch.cuda.synchronize()
+    assert not bool((p.tmpVec[N[0] // 2:2 * N[0] - N[0] // 2] == 7.0).all())
+    orc = O.OraclePlan(k, N, m=4, sigma=2.0, precompute=O.LINEAR)
+    assert rel(x, orc.adjoint(fh.cpu().numpy())) < 1e-12
