@@ -281,7 +281,7 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
   char* tile = smem + G::TILE_OFF;
   char* lrec = smem + G::REC_OFF;
   C* fv = (C*)(smem + G::FV_OFF);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
 
   // the first batch's records; the item's node values, held in registers
   // (thread t holds nodes it.y + t + h NT)
@@ -331,13 +331,33 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
     }
     __syncthreads();
     if (b0 + G::NB < it.z) rs.load(a, b0 + G::NB, min(G::NB, it.z - b0 - G::NB));
+    // z weights and cell of the wavefront's node from the record in global
+    // memory (uniform: scalar loads, one node ahead), x and y weights per
+    // lane from the LDS copy
+    using CR = const __attribute__((address_space(4))) R;
+    using CU = const __attribute__((address_space(4))) unsigned;
+    const char* gr = a.rec + (size_t)b0 * G::RB;
+    R w3n[W];
+    unsigned celln = 0;
+    if (wave < nb) {
+      CR* g = (CR*)(gr + (size_t)wave * G::RB);
+#pragma unroll
+      for (int l = 0; l < W; ++l) w3n[l] = g[2 * W + l];
+      celln = ((CU*)(g + 3 * W))[0];
+    }
     for (int n = wave; n < nb; n += NW) {
       const C f = fv[n];
       const R* w = rec_w<R, 3, W>(lrec, n);
       R w3[W];
 #pragma unroll
-      for (int l = 0; l < W; ++l) w3[l] = w[2 * W + l];
-      const int cell = (int)rec_loc<R, 3, W>(lrec, n);
+      for (int l = 0; l < W; ++l) w3[l] = w3n[l];
+      const int cell = (int)celln;
+      if (n + NW < nb) {
+        CR* g = (CR*)(gr + (size_t)(n + NW) * G::RB);
+#pragma unroll
+        for (int l = 0; l < W; ++l) w3n[l] = g[2 * W + l];
+        celln = ((CU*)(g + 3 * W))[0];
+      }
 #pragma unroll
       for (int t0l = 0; t0l < W * W; t0l += 64) {
         const int t = t0l + lane;
