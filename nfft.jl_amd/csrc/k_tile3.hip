@@ -70,14 +70,19 @@ __device__ __forceinline__ int wrap1(int x, int n) { return x < 0 ? x + n : (x >
 // dimension, m <= T): per dimension the padded range splits into the left
 // halo (m - 1 cells, belonging to the previous tile), the interior (T cells)
 // and the right halo (m cells, belonging to the next tile); the 27 products
-// are stored as contiguous x-fastest blocks, the interior block first, so
-// that the fold reads every block once and in full.
+// are stored as contiguous blocks, the interior block first, each ordered
+// x fastest, then z, then y (so that one y row of a block, as the fold fused
+// into the first adjoint FFT pass reads it, is one contiguous run).
 template <int W, int T, int TZ>
 struct Blocks3 {
   static constexpr int m = W / 2;
   static constexpr __host__ __device__ int ext(int i, int t = T) { return i == 0 ? m - 1 : i == 1 ? t : m; }
   static constexpr __host__ __device__ int org(int i, int t = T) { return i == 0 ? 0 : i == 1 ? m - 1 : m - 1 + t; }
   static constexpr __host__ __device__ int size(int b) { return ext(b % 3) * ext((b / 3) % 3) * ext(b / 9, TZ); }
+  // position of cell (x, y, z) in block b
+  static constexpr __host__ __device__ int at(int b, int x, int y, int z) {
+    return x + ext(b % 3) * (z + ext(b / 9, TZ) * y);
+  }
   static constexpr __host__ __device__ int off(int b) {
     if (b == 13) return 0;
     int o = size(13);
@@ -403,10 +408,12 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
       constexpr int ox = B::org(b % 3), oy = B::org((b / 3) % 3), oz = B::org(b / 9, G::TZ);
       constexpr int off = B::off(b), n = ex * ey * ez;
       C* dst = out + off;
-      for (int i = threadIdx.x; i < n; i += NT) {
-        const int lx = i % ex, q = i / ex;
-        const int ly = q % ey, lz = q / ey;
-        dst[i] = SpreadCell<R>::get(tile, G::CELLS, (ox + lx) + G::SX * (oy + ly) + G::SXY * (oz + lz), inv);
+      if constexpr (n > 0) {
+        for (int i = threadIdx.x; i < n; i += NT) {
+          const int lx = i % ex, q = i / ex;
+          const int lz = q % ez, ly = q / ez;
+          dst[i] = SpreadCell<R>::get(tile, G::CELLS, (ox + lx) + G::SX * (oy + ly) + G::SXY * (oz + lz), inv);
+        }
       }
     });
     return;
@@ -543,7 +550,7 @@ __global__ __launch_bounds__(256) void k_fold3b(KArgs a, typename cpx<R>::T* __r
     constexpr int bx = 1 - dx, by = 1 - dy, bz = 1 - dz;
     constexpr int sx = dx == 1 ? T - m + 1 : 0, sy = dy == 1 ? T - m + 1 : 0, sz = dz == 1 ? TZ - m + 1 : 0;
     constexpr int zlo = dz == 1 ? TZ - m + 1 : 0, zhi = dz == -1 ? m : TZ;
-    constexpr int ex = B::ext(bx), ey = B::ext(by);
+    constexpr int ex = B::ext(bx);
     constexpr int off = B::off(bx + 3 * by + 9 * bz);
     const bool in = mine && (dy == 0 || (dy == -1 ? ly < m : ly >= T - m + 1)) &&
                     (dx == 0 || (dx == -1 ? lx < m : lx >= T - m + 1));
@@ -551,11 +558,11 @@ __global__ __launch_bounds__(256) void k_fold3b(KArgs a, typename cpx<R>::T* __r
     const int t = tx + ntx * (ty + nty * tz);
     const int i0 = a.tstart[t], i1 = a.tstart[t + 1];
     for (int itm = i0; itm < i1; ++itm) {
-      const C* s = scr + (size_t)itm * G::SLOT + off + (lx - sx) + ex * (ly - sy);
+      const C* s = scr + (size_t)itm * G::SLOT + off + B::at(bx + 3 * by + 9 * bz, lx - sx, ly - sy, 0);
 #pragma unroll
       for (int k = zlo; k < zhi; ++k) {
         if (in) {
-          const C v = s[ex * ey * (k - sz)];
+          const C v = s[ex * (k - sz)];
           acc[k].x += v.x;
           acc[k].y += v.y;
         }
@@ -624,13 +631,12 @@ __global__ __launch_bounds__((lf::LineGeo<R, L>::NT)) void k_fold_fft3(KArgs a, 
     constexpr int bx = 1 - dx, by = 1 - dy, bz = 1 - dz;
     constexpr int sx = dx == 1 ? T - m + 1 : 0, sy = dy == 1 ? T - m + 1 : 0, sz = dz == 1 ? TZ - m + 1 : 0;
     constexpr int zlo = dz == 1 ? TZ - m + 1 : 0, zhi = dz == -1 ? m : TZ;
-    constexpr int ex = B::ext(bx), ey = B::ext(by);
     constexpr int off = B::off(bx + 3 * by + 9 * bz);
     if (dy == 0 || (dy == -1 ? ly < m : ly >= T - m + 1)) {
       const bool use = (dx == 0 || (dx == -1 ? lx < m : lx >= T - m + 1)) && lz >= zlo && lz < zhi;
       if (use) {
         const int* row = item[(dy + 1) * 3 + dx + 1];
-        const unsigned li = off + (lx - sx) + ex * (ly - sy) + ex * ey * (lz - sz);
+        const unsigned li = off + B::at(bx + 3 * by + 9 * bz, lx - sx, ly - sy, lz - sz);
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
           const int it = row[(q + u * (TL / TZ) + dz + NZ) % NZ];
