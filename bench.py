@@ -323,6 +323,8 @@ def run_tools(args, torch, nfftm, dev):
                      "frac": bytes_prod / res["f32"]["t_toep"] / 1e9 / 8000.0, "traffic": None},
         "cpu_baseline": cpu,
     }
+    if sparse is not None:
+        out["sparse_matrix_baseline"] = sparse
     print(json.dumps(out))
 
 
@@ -339,6 +341,9 @@ def main():
                     help="autotune the tile shape of both plans before timing (plan set-up)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (reference kernel for HBM counters)")
+    ap.add_argument("--sparse-baseline", action="store_true",
+                    help="also time the reference GPU backend's algorithm on the same workload: "
+                         "precompute FULL as a device sparse matrix (FP32)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -432,6 +437,41 @@ def main():
     torch.cuda.synchronize()
     t64 = (time.perf_counter() - t0) / n64
 
+    # the reference GPU backend's algorithm (FULL weights as a device sparse
+    # matrix, gather / atomic-scatter products) on the same workload
+    sparse = None
+    if args.sparse_baseline:
+        log("bench: sparse-matrix baseline (FP32, precompute FULL)")
+        t_build = time.perf_counter()
+        psp = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=nfftm.FULL, dtype=torch.float32,
+                              sparse_matrix=True)
+        torch.cuda.synchronize()
+        t_build = time.perf_counter() - t_build
+        ysp = torch.empty_like(y32)
+        xsp = torch.empty_like(x32)
+
+        def stepsp():
+            psp.forward(f32, ysp)
+            psp.adjoint(fh32, xsp)
+
+        stepsp()
+        torch.cuda.synchronize()
+        nsp = max(3, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(nsp):
+            stepsp()
+        torch.cuda.synchronize()
+        tsp = (time.perf_counter() - t0) / nsp
+        step64()
+        torch.cuda.synchronize()
+        sparse = {"value": M / tsp, "ms_per_step": tsp * 1e3, "steps": nsp, "plan_s": t_build,
+                  "matrix_bytes": M * (2 * m) ** D * 8,
+                  "fwd_vs_f64": float(torch.linalg.norm(ysp.to(torch.complex128) - y64) / torch.linalg.norm(y64)),
+                  "adj_vs_f64": float(torch.linalg.norm(xsp.to(torch.complex128) - x64) / torch.linalg.norm(x64)),
+                  "speedup_of_default": tsp / t32}
+        del psp, ysp, xsp
+        torch.cuda.empty_cache()
+
     # accuracy: FP32 vs FP64 plan (full), and vs the FP64 NDFT on samples
     step32()
     step64()
@@ -520,6 +560,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if sparse is not None:
+        out["sparse_matrix_baseline"] = sparse
     print(json.dumps(out))
     if args.copy_calib:
         # a device-to-device copy of exactly 256 MiB read + 256 MiB written:

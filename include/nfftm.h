@@ -81,12 +81,21 @@ enum {
                               the grid (nfftm_grid) then holds the full FFT
                               after a forward transform and a partial one
                               after an adjoint. */
-  NFFTM_FLAG_SEPARATE_FOLD = 16 /* keep the 3-D adjoint's fold and its first
+  NFFTM_FLAG_SEPARATE_FOLD = 16, /* keep the 3-D adjoint's fold and its first
                               pruned FFT pass as two kernels.  By default they
                               are one kernel where every tile holds at most
                               one work item and the tile shape matches the
                               pass (e.g. 256-point z lines): bit-identical
                               results, the folded grid is never written. */
+  NFFTM_FLAG_SPARSE = 32    /* convolution as a device sparse matrix, the
+                              reference GPU backend's form of precompute =
+                              FULL (ext/NFFTGPUArraysExt): at nodes! the plan
+                              stores the W^D window weights (of its precompute
+                              mode) and grid indices of every node, 4 + s_r
+                              bytes each; forward = gather product, adjoint =
+                              atomic scatter.  A comparison baseline: slower
+                              than the default tiled kernels and
+                              M W^D (4 + s_r) bytes of device memory. */
 };
 
 /* Plan creation: replaces NFFTPlan(k, N; m, sigma, window, precompute,

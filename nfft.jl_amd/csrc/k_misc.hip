@@ -245,6 +245,134 @@ static hipError_t generic_w(bool spread, int W, const KArgs& a, hipStream_t s, c
   return hipGetLastError();
 }
 
+// Sparse-matrix convolution: the reference's precompute = FULL form, whose
+// GPU backend holds B as a device sparse matrix (src/precomputation.jl:81-120,
+// ext/NFFTGPUArraysExt/implementation.jl:3-136): row j holds the K = W^D
+// window weights of node j (the plan's precompute mode, normalised as the
+// generic kernels) and their grid indices, rows of fixed length (CSR without
+// a row-pointer array).  Forward: fhat_j = sum_l B_jl g_l, a wavefront per
+// row.  Adjoint: g_l += B_jl fhat_j by global atomics, a wavefront per row.
+template <typename R, int W>
+__global__ __launch_bounds__(256) void k_sparse_build(KArgs a, int* __restrict__ cols, R* __restrict__ vals) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= a.M) return;
+  R w[MAXD][W];
+  long long base[MAXD];
+  for (int d = 0; d < a.D; ++d) {
+    const int cw = node_weights<R, W>(a, d, a.ks[(size_t)d * a.M + j], w[d]);
+    base[d] = (long long)cw - a.m + 1;
+  }
+  int l[MAXD];
+  for (int d = 0; d < MAXD; ++d) l[d] = 0;
+  size_t o = (size_t)j;
+  for (int d = 0; d < a.D; ++d) o *= W;
+  while (true) {
+    R pw = 1;
+    long long gi = 0;
+    for (int d = a.D - 1; d >= 0; --d) {
+      pw = pw * w[d][l[d]];
+      gi += wrapidx(base[d] + l[d], a.Nt[d]) * a.gstr[d];
+    }
+    cols[o] = (int)gi;
+    vals[o] = pw;
+    ++o;
+    int d = 0;
+    for (; d < a.D; ++d) {
+      if (++l[d] < W) break;
+      l[d] = 0;
+    }
+    if (d == a.D) break;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ R wave_sum64(R v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_sparse_interp(KArgs a, int K, const int* __restrict__ cols,
+                                                       const R* __restrict__ vals,
+                                                       const typename cpx<R>::T* __restrict__ grid,
+                                                       typename cpx<R>::T* __restrict__ fhat) {
+  using C = typename cpx<R>::T;
+  const int j = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (j >= a.M) return;  // whole wavefronts
+  const size_t o = (size_t)j * K;
+  R sr = 0, si = 0;
+  for (int l = lane; l < K; l += 64) {
+    const R v = vals[o + l];
+    const C g = grid[cols[o + l]];
+    sr = fma(v, g.x, sr);
+    si = fma(v, g.y, si);
+  }
+  sr = wave_sum64(sr);
+  si = wave_sum64(si);
+  if (lane == 0) fhat[j] = C{sr * (R)a.oscale, si * (R)a.oscale};
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_sparse_spread(KArgs a, int K, const int* __restrict__ cols,
+                                                       const R* __restrict__ vals,
+                                                       const typename cpx<R>::T* __restrict__ fhat,
+                                                       typename cpx<R>::T* __restrict__ grid) {
+  using C = typename cpx<R>::T;
+  const int j = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (j >= a.M) return;
+  const size_t o = (size_t)j * K;
+  const C f = fhat[j];
+  const R fr = f.x * (R)a.oscale, fi = f.y * (R)a.oscale;
+  for (int l = lane; l < K; l += 64) {
+    const R v = vals[o + l];
+    R* g = (R*)(grid + cols[o + l]);
+    atomicAdd(g, v * fr);
+    atomicAdd(g + 1, v * fi);
+  }
+}
+
+template <typename R>
+static hipError_t sparse_build_w(int W, const KArgs& a, hipStream_t s, int* cols, void* vals) {
+  const dim3 grid((a.M + 255) / 256), blk(256);
+#define NFFTM_SB(WW)                                                                       \
+  case WW:                                                                                 \
+    hipLaunchKernelGGL((k_sparse_build<R, WW>), grid, blk, 0, s, a, cols, (R*)vals);      \
+    break;
+  switch (W) {
+    NFFTM_SB(2) NFFTM_SB(4) NFFTM_SB(6) NFFTM_SB(8) NFFTM_SB(10) NFFTM_SB(12) NFFTM_SB(14) NFFTM_SB(16)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef NFFTM_SB
+  return hipGetLastError();
+}
+
+hipError_t launch_sparse_build(bool dbl, int W, const KArgs& a, hipStream_t s, int* cols, void* vals) {
+  if (a.M == 0) return hipSuccess;
+  return dbl ? sparse_build_w<double>(W, a, s, cols, vals) : sparse_build_w<float>(W, a, s, cols, vals);
+}
+
+hipError_t launch_sparse(bool spread, bool dbl, const KArgs& a, int K, const int* cols, const void* vals,
+                         hipStream_t s, const void* in, void* out) {
+  if (a.M == 0) return hipSuccess;
+  const dim3 grid((unsigned)((a.M + 3) / 4)), blk(256);
+  if (dbl) {
+    using C = cpx<double>::T;
+    if (spread)
+      hipLaunchKernelGGL((k_sparse_spread<double>), grid, blk, 0, s, a, K, cols, (const double*)vals, (const C*)in, (C*)out);
+    else
+      hipLaunchKernelGGL((k_sparse_interp<double>), grid, blk, 0, s, a, K, cols, (const double*)vals, (const C*)in, (C*)out);
+  } else {
+    using C = cpx<float>::T;
+    if (spread)
+      hipLaunchKernelGGL((k_sparse_spread<float>), grid, blk, 0, s, a, K, cols, (const float*)vals, (const C*)in, (C*)out);
+    else
+      hipLaunchKernelGGL((k_sparse_interp<float>), grid, blk, 0, s, a, K, cols, (const float*)vals, (const C*)in, (C*)out);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_generic(bool spread, bool dbl, int W, const KArgs& a, hipStream_t s, const void* in,
                           void* out) {
   if (a.M == 0) return hipSuccess;

@@ -115,6 +115,12 @@ hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s);
 
 // Sum the per-item padded tiles in a.scratch into the grid (D <= 3).
 hipError_t launch_fold(int D, bool dbl, int W, const KArgs& a, hipStream_t s, void* grid);
+// Sparse-matrix convolution (NFFTM_FLAG_SPARSE): build the M x W^D rows of
+// window weights and grid indices, then forward (interpolation) or adjoint
+// (spreading by atomics into a zeroed grid) products with them.
+hipError_t launch_sparse_build(bool dbl, int W, const KArgs& a, hipStream_t s, int* cols, void* vals);
+hipError_t launch_sparse(bool spread, bool dbl, const KArgs& a, int K, const int* cols, const void* vals,
+                         hipStream_t s, const void* in, void* out);
 hipError_t launch_fold_d3(bool dbl, int W, const KArgs& a, hipStream_t s, void* grid);
 
 // Compile-time tile geometry of the 3-D kernels: tile edge T, padded edge P,

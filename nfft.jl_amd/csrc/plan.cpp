@@ -411,6 +411,11 @@ struct nfftm_plan_s {
   DevBuf<int4> d_items;
   DevBuf<int> d_tstart;
   DevBuf<char> d_scratch, d_rec;
+  // NFFTM_FLAG_SPARSE: rows of W^D grid indices and window weights per node
+  bool sparse = false;
+  int sparseK = 0;
+  DevBuf<int> d_bcols;
+  DevBuf<char> d_bvals;
   int nitems = 0;
   bool single_items = false;  // tiled, nitems > 0, no tile with two work items
   int gen = 0;        // incremented whenever the node arrays are rebuilt
@@ -814,6 +819,18 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
     HIPCHK(hipMemcpy(p.d_perm.p, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice));
   if (!items.empty())
     HIPCHK(hipMemcpy(p.d_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+  if (p.sparse && M > 0) {
+    i64 K = 1;
+    for (int d = 0; d < D; ++d) K *= p.W;
+    const double bytes = (double)M * (double)K * (4.0 + (p.dbl() ? 8.0 : 4.0));
+    if (K > 0x7fffffff || bytes > 128.0 * (1ull << 30))
+      return fail(NFFTM_ERR_UNSUPPORTED, "sparse matrix of %lld x %lld entries (%.1f GB) is too large", (long long)M,
+                  (long long)K, bytes / 1e9);
+    p.sparseK = (int)K;
+    HIPCHK(p.d_bcols.alloc((size_t)M * K));
+    HIPCHK(p.d_bvals.alloc((size_t)M * K * (p.dbl() ? 8 : 4)));
+    HIPCHK(launch_sparse_build(p.dbl(), p.W, p.kargs(), p.stream, p.d_bcols.p, p.d_bvals.p));
+  }
   if (p.tiled) {
     HIPCHK(p.d_tstart.alloc(tstart.size()));
     HIPCHK(hipMemcpy(p.d_tstart.p, tstart.data(), tstart.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -913,6 +930,8 @@ int run_conv(nfftm_plan_s& p, bool spread, const void* in, void* out, bool stand
     ProfScope ps(p, spread ? 4 : 2);
     if (p.tiled)
       e = launch_tile(p.D, spread, p.dbl(), p.W, p.NT, a, p.lds, p.stream, in, spread ? nullptr : out);
+    else if (p.sparse)
+      e = launch_sparse(spread, p.dbl(), a, p.sparseK, p.d_bcols.p, p.d_bvals.p, p.stream, in, out);
     else
       e = launch_generic(spread, p.dbl(), p.W, a, p.stream, in, out);
   }
@@ -1021,7 +1040,8 @@ int create_impl(nfftm_plan_s& p, const double* k) {
       return fail(NFFTM_ERR_UNSUPPORTED, "oversampled size %lld exceeds 2^31-1", (long long)p.Nt[d]);
   if (p.Q / 2 > 0x7fffffffLL)
     return fail(NFFTM_ERR_UNSUPPORTED, "oversampled grid of %lld points exceeds 2^32", (long long)p.Q);
-  p.tiled = (p.flags & NFFTM_FLAG_BLOCKING) && p.D <= 3;
+  p.sparse = (p.flags & NFFTM_FLAG_SPARSE) != 0;
+  p.tiled = (p.flags & NFFTM_FLAG_BLOCKING) && p.D <= 3 && !p.sparse;
   if (p.tiled) {
     choose_tiles(p);
     p.tiled = tiles_ok(p);

@@ -257,7 +257,8 @@ class NFFTPlan:
         Nj = (ctypes.c_int64 * D)(*N[::-1])
         flags = ((_L.FLAG_BLOCKING if blocking else 0) | (_L.FLAG_GRAPHS if kw.get("graphs", False) else 0)
                  | (_L.FLAG_ROCFFT if kw.get("rocfft", False) else 0)
-                 | (0 if kw.get("fused_fold", True) else _L.FLAG_SEPARATE_FOLD))
+                 | (0 if kw.get("fused_fold", True) else _L.FLAG_SEPARATE_FOLD)
+                 | (_L.FLAG_SPARSE if kw.get("sparse_matrix", False) else 0))
         _check(self._lib.nfftm_plan_create(
             ctypes.byref(self._handle), prec, D, Nj, k.shape[1],
             kj.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m, sigma, _WINDOWS[window],

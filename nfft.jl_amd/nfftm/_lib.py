@@ -15,7 +15,7 @@ OK, ERR_ARG, ERR_DIM, ERR_UNSUPPORTED, ERR_HIP, ERR_FFT, ERR_OOM = range(7)
 F32, F64, MIXED = 0, 1, 2
 FULL, TENSOR, LINEAR, POLYNOMIAL = 1, 2, 3, 4
 KAISER_BESSEL = 0
-FLAG_BLOCKING, FLAG_TIMING, FLAG_GRAPHS, FLAG_ROCFFT, FLAG_SEPARATE_FOLD = 1, 2, 4, 8, 16
+FLAG_BLOCKING, FLAG_TIMING, FLAG_GRAPHS, FLAG_ROCFFT, FLAG_SEPARATE_FOLD, FLAG_SPARSE = 1, 2, 4, 8, 16, 32
 BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
 
 # every symbol include/nfftm.h declares
