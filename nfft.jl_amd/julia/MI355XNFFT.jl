@@ -38,6 +38,10 @@ const NFFTM_F32 = Cint(0)
 const NFFTM_F64 = Cint(1)
 const NFFTM_FLAG_BLOCKING = Cint(1)
 const NFFTM_FLAG_TIMING = Cint(2)
+const NFFTM_FLAG_GRAPHS = Cint(4)
+const NFFTM_FLAG_ROCFFT = Cint(8)
+const NFFTM_FLAG_SEPARATE_FOLD = Cint(16)
+const NFFTM_FLAG_SPARSE = Cint(32)
 
 # Error classes as the reference raises them (src/utils.jl:46-55, :98-105).
 function check(rc::Cint)
@@ -88,13 +92,16 @@ end
 function AbstractNFFTs.plan_nfft(::MI355XBackend, ::Type{<:ROCArray}, k::Matrix{T}, N::NTuple{D,Int};
     dims::Union{Integer,UnitRange{Int64}}=1:D, fftflags=nothing, sortNodes=false,
     blocking=true, window=:kaiser_bessel, precompute=AbstractNFFTs.POLYNOMIAL,
-    timing::Union{Nothing,TimingStats}=nothing, kargs...) where {T<:Union{Float32,Float64},D}
+    timing::Union{Nothing,TimingStats}=nothing, graphs=false, rocfft=false, fused_fold=true,
+    sparse_matrix=false, kargs...) where {T<:Union{Float32,Float64},D}
   dims != 1:D && error("MI355X NFFT does not work along directions right now!")
   reltol, m, σ = AbstractNFFTs.accuracyParams(; kargs...)
   sortNodes && (k = sortslices(k, dims=2))
   kd = Matrix{Float64}(k)
   h = Ref{Ptr{Cvoid}}(C_NULL)
-  flags = (blocking ? NFFTM_FLAG_BLOCKING : Cint(0)) | (timing === nothing ? Cint(0) : NFFTM_FLAG_TIMING)
+  flags = (blocking ? NFFTM_FLAG_BLOCKING : Cint(0)) | (timing === nothing ? Cint(0) : NFFTM_FLAG_TIMING) |
+          (graphs ? NFFTM_FLAG_GRAPHS : Cint(0)) | (rocfft ? NFFTM_FLAG_ROCFFT : Cint(0)) |
+          (fused_fold ? Cint(0) : NFFTM_FLAG_SEPARATE_FOLD) | (sparse_matrix ? NFFTM_FLAG_SPARSE : Cint(0))
   t = @elapsed check(ccall((:nfftm_plan_create, libnfftm), Cint,
       (Ref{Ptr{Cvoid}}, Cint, Cint, Ptr{Int64}, Int64, Ptr{Float64}, Cint, Cdouble, Cint, Cint, Cint, Ptr{Cvoid}),
       h, precision_of(T), D, collect(Int64, N), size(k, 2), kd, m, σ, window_id(window),

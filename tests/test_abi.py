@@ -148,3 +148,16 @@ def test_timingstats_fields():
     assert [f for f in t.__dataclass_fields__] == ["pre", "conv", "fft", "deconv", "conv_adjoint",
                                                    "fft_adjoint", "deconv_adjoint"]
     assert str(t).startswith("Timing: pre = 0.0000 s")
+
+
+def test_flag_values_agree_across_bindings():
+    """The plan flags of include/nfftm.h, the Python mirror and the Julia
+    binding carry the same values."""
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(REPO, "include", "nfftm.h")).read(), flags=re.S)
+    hdr = {k: int(v) for k, v in re.findall(r"\bNFFTM_FLAG_([A-Z_]+)\s*=\s*(\d+)", txt)}
+    assert set(hdr) == {"BLOCKING", "TIMING", "GRAPHS", "ROCFFT", "SEPARATE_FOLD", "SPARSE"}
+    for k, v in hdr.items():
+        assert getattr(_lib, "FLAG_" + k) == v, k
+    jl = open(os.path.join(REPO, "nfft.jl_amd", "julia", "MI355XNFFT.jl")).read()
+    jlf = {k: int(v) for k, v in re.findall(r"const NFFTM_FLAG_([A-Z_]+) = Cint\((\d+)\)", jl)}
+    assert jlf == hdr
