@@ -47,7 +47,7 @@ HBM_PEAK = 8.0e12  # B/s, MI355X
 # profiles/nfft3d_128cubed_M2p21_hbm_traffic.md holds the passes and the
 # command.  Keyed by (workload, stage) for the default FP32 plan of that
 # workload (3-D: k_spread3<float, 8, 1024, 1>, flat 16 x 16 x 8 tiles).
-MEASURED_TRAFFIC = {("3d", "spreading"): 1076825119, ("3d", "interpolation"): 564038470}
+MEASURED_TRAFFIC = {("3d", "spreading"): 1075470013, ("3d", "interpolation"): 564103562}
 
 STAGES = ("deconvolve", "fft_forward", "interpolation", "grid_clear", "spreading", "fft_backward",
           "deconvolve_transpose", "node_gradient")
