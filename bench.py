@@ -441,36 +441,40 @@ def main():
     # matrix, gather / atomic-scatter products) on the same workload
     sparse = None
     if args.sparse_baseline:
-        log("bench: sparse-matrix baseline (FP32, precompute FULL)")
-        t_build = time.perf_counter()
-        psp = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=nfftm.FULL, dtype=torch.float32,
-                              sparse_matrix=True)
-        torch.cuda.synchronize()
-        t_build = time.perf_counter() - t_build
-        ysp = torch.empty_like(y32)
-        xsp = torch.empty_like(x32)
+        sparse = {}
+        for name, rdt, cdt, fin, fhin in (("f32", torch.float32, torch.complex64, f32, fh32),
+                                          ("f64", torch.float64, torch.complex128, f64, fh64)):
+            log(f"bench: sparse-matrix baseline ({name}, precompute FULL)")
+            t_build = time.perf_counter()
+            psp = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=nfftm.FULL, dtype=rdt,
+                                  sparse_matrix=True)
+            torch.cuda.synchronize()
+            t_build = time.perf_counter() - t_build
+            ysp = torch.empty(M, dtype=cdt, device=dev)
+            xsp = torch.empty(N, dtype=cdt, device=dev)
 
-        def stepsp():
-            psp.forward(f32, ysp)
-            psp.adjoint(fh32, xsp)
+            def stepsp():
+                psp.forward(fin, ysp)
+                psp.adjoint(fhin, xsp)
 
-        stepsp()
-        torch.cuda.synchronize()
-        nsp = max(3, args.steps // 4)
-        t0 = time.perf_counter()
-        for _ in range(nsp):
             stepsp()
-        torch.cuda.synchronize()
-        tsp = (time.perf_counter() - t0) / nsp
-        step64()
-        torch.cuda.synchronize()
-        sparse = {"value": M / tsp, "ms_per_step": tsp * 1e3, "steps": nsp, "plan_s": t_build,
-                  "matrix_bytes": M * (2 * m) ** D * 8,
-                  "fwd_vs_f64": float(torch.linalg.norm(ysp.to(torch.complex128) - y64) / torch.linalg.norm(y64)),
-                  "adj_vs_f64": float(torch.linalg.norm(xsp.to(torch.complex128) - x64) / torch.linalg.norm(x64)),
-                  "speedup_of_default": tsp / t32}
-        del psp, ysp, xsp
-        torch.cuda.empty_cache()
+            torch.cuda.synchronize()
+            nsp = max(3, args.steps // 4)
+            t0 = time.perf_counter()
+            for _ in range(nsp):
+                stepsp()
+            torch.cuda.synchronize()
+            tsp = (time.perf_counter() - t0) / nsp
+            step64()
+            torch.cuda.synchronize()
+            sparse[name] = {
+                "value": M / tsp, "ms_per_step": tsp * 1e3, "steps": nsp, "plan_s": t_build,
+                "matrix_bytes": M * (2 * m) ** D * (4 + (8 if name == "f64" else 4)),
+                "fwd_vs_f64": float(torch.linalg.norm(ysp.to(torch.complex128) - y64) / torch.linalg.norm(y64)),
+                "adj_vs_f64": float(torch.linalg.norm(xsp.to(torch.complex128) - x64) / torch.linalg.norm(x64)),
+                "default_f32_over_this": tsp / t32}
+            del psp, ysp, xsp
+            torch.cuda.empty_cache()
 
     # accuracy: FP32 vs FP64 plan (full), and vs the FP64 NDFT on samples
     step32()
