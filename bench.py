@@ -323,8 +323,6 @@ def run_tools(args, torch, nfftm, dev):
                      "frac": bytes_prod / res["f32"]["t_toep"] / 1e9 / 8000.0, "traffic": None},
         "cpu_baseline": cpu,
     }
-    if sparse is not None:
-        out["sparse_matrix_baseline"] = sparse
     print(json.dumps(out))
 
 
