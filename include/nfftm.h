@@ -147,6 +147,26 @@ int nfftm_convolve(nfftm_plan p, const void* g_dev, void* fhat_dev);
  * g is overwritten. */
 int nfftm_convolve_transpose(nfftm_plan p, const void* fhat_dev, void* g_dev);
 
+/* Element types of caller buffers for the typed stage operators below. */
+enum { NFFTM_R32 = 0, NFFTM_R64 = 1, NFFTM_C32 = 2, NFFTM_C64 = 3, NFFTM_R16 = 4, NFFTM_C16 = 5 };
+
+/* convolve!(p, g, fHat) and convolve_transpose!(p, fHat, g) on buffers of any
+ * real or complex element type (half, single or double), as the reference's
+ * methods take RealOrComplex arrays of any precision: src/convolution.jl:20-53
+ * and :115-149, exercised by test/convolve.jl:43-146, and called with real
+ * weights and grids by NFFTTools.sdc! (NFFTTools/src/samplingDensity.jl:106,
+ * 112).  The sums run in the plan's precision; real inputs give real results
+ * (the window is real).  Checks, in the reference's order: complex input with
+ * a real output -> NFFTM_ERR_ARG (src/convolution.jl:48-53, :144-149); a
+ * buffer length (elements, not bytes) other than prod(Ntilde) for g or M for
+ * fHat -> NFFTM_ERR_DIM (:26-29, :121-124).  A buffer of another type than
+ * the plan's complex type is staged through the plan's grid or a plan-owned
+ * vector of M values (allocated at the first such call). */
+int nfftm_convolve_typed(nfftm_plan p, const void* g_dev, int g_type, int64_t g_len, void* fhat_dev,
+                         int fhat_type, int64_t fhat_len);
+int nfftm_convolve_transpose_typed(nfftm_plan p, const void* fhat_dev, int fhat_type, int64_t fhat_len,
+                                   void* g_dev, int g_type, int64_t g_len);
+
 /* Oversampled FFT stage on a caller grid, in place, unnormalised:
  * p.forwardFFT * g (sign -1) or p.backwardFFT * g (sign +1),
  * src/implementation.jl:87-88, 161, 182.  direction: -1 or +1. */

@@ -104,6 +104,41 @@ __global__ __launch_bounds__(RB) void k_cpx_to_real(const C* __restrict__ wc, R*
     w[i] = wc[i].x * c;
 }
 
+// Element-type conversion of a vector of n values (ELT_* codes): real to
+// complex with a zero imaginary part, complex to real by the real part, and
+// any precision change by one rounding (through double).
+template <int T>
+struct Elt;
+template <>
+struct Elt<ELT_R16> { using V = _Float16; static constexpr bool CPX = false; };
+template <>
+struct Elt<ELT_R32> { using V = float; static constexpr bool CPX = false; };
+template <>
+struct Elt<ELT_R64> { using V = double; static constexpr bool CPX = false; };
+template <>
+struct Elt<ELT_C16> { using V = _Float16; static constexpr bool CPX = true; };
+template <>
+struct Elt<ELT_C32> { using V = float; static constexpr bool CPX = true; };
+template <>
+struct Elt<ELT_C64> { using V = double; static constexpr bool CPX = true; };
+
+template <int TI, int TO>
+__global__ __launch_bounds__(RB) void k_convert(const typename Elt<TI>::V* __restrict__ in,
+                                                typename Elt<TO>::V* __restrict__ out, long long n) {
+  using VI = typename Elt<TI>::V;
+  using VO = typename Elt<TO>::V;
+  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < n; i += (long long)gridDim.x * RB) {
+    const VI re = Elt<TI>::CPX ? in[2 * i] : in[i];
+    const VI im = Elt<TI>::CPX ? in[2 * i + 1] : (VI)0;
+    if constexpr (Elt<TO>::CPX) {
+      out[2 * i] = (VO)(double)re;
+      out[2 * i + 1] = (VO)(double)im;
+    } else {
+      out[i] = (VO)(double)re;
+    }
+  }
+}
+
 template <typename C, typename R>
 __global__ __launch_bounds__(RB) void k_fill_one(C* __restrict__ u, long long n) {
   for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < n; i += (long long)gridDim.x * RB)
@@ -296,6 +331,36 @@ hipError_t launch_shift_sign(bool dbl, void* x, const ToepArgs& a, hipStream_t s
     hipLaunchKernelGGL((k_shift_sign<double2, double>), dim3(grid_blocks(n)), dim3(RB), 0, s, (double2*)x, a);
   else hipLaunchKernelGGL((k_shift_sign<float2, float>), dim3(grid_blocks(n)), dim3(RB), 0, s, (float2*)x, a);
   return hipGetLastError();
+}
+
+template <int TI>
+static hipError_t convert_to(int tout, const void* in, void* out, long long n, hipStream_t s, unsigned nb) {
+#define NFFTM_CV(TO)                                                                                  \
+  case TO:                                                                                            \
+    hipLaunchKernelGGL((k_convert<TI, TO>), dim3(nb), dim3(RB), 0, s, (const typename Elt<TI>::V*)in, \
+                       (typename Elt<TO>::V*)out, n);                                                 \
+    break;
+  switch (tout) {
+    NFFTM_CV(ELT_R32) NFFTM_CV(ELT_R64) NFFTM_CV(ELT_C32) NFFTM_CV(ELT_C64) NFFTM_CV(ELT_R16) NFFTM_CV(ELT_C16)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef NFFTM_CV
+  return hipGetLastError();
+}
+
+hipError_t launch_convert(int tin, const void* in, int tout, void* out, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned nb = (unsigned)grid_blocks(n);
+  switch (tin) {
+    case ELT_R32: return convert_to<ELT_R32>(tout, in, out, n, s, nb);
+    case ELT_R64: return convert_to<ELT_R64>(tout, in, out, n, s, nb);
+    case ELT_C32: return convert_to<ELT_C32>(tout, in, out, n, s, nb);
+    case ELT_C64: return convert_to<ELT_C64>(tout, in, out, n, s, nb);
+    case ELT_R16: return convert_to<ELT_R16>(tout, in, out, n, s, nb);
+    case ELT_C16: return convert_to<ELT_C16>(tout, in, out, n, s, nb);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace nfftm

@@ -63,6 +63,10 @@ hipError_t launch_sdc_update(bool dbl, void* wc, const void* wt, long long n, co
 hipError_t launch_real_to_cpx(bool dbl, const void* w, void* wc, long long n, hipStream_t s);
 hipError_t launch_cpx_to_real(bool dbl, const void* wc, void* w, long long n, const double* sc, hipStream_t s);
 hipError_t launch_fill_one(bool dbl, void* u, long long n, hipStream_t s);
+// Element types of caller buffers (the values of the C ABI's NFFTM_R32 ...
+// NFFTM_C16) and the conversion of n values between any two of them.
+enum { ELT_R32 = 0, ELT_R64 = 1, ELT_C32 = 2, ELT_C64 = 3, ELT_R16 = 4, ELT_C16 = 5 };
+hipError_t launch_convert(int tin, const void* in, int tout, void* out, long long n, hipStream_t s);
 hipError_t launch_mul_real(bool dbl, void* f, const void* wc, long long n, hipStream_t s);
 hipError_t launch_pad_corner(bool dbl, const void* y, void* x, const ToepArgs& a, hipStream_t s);
 hipError_t launch_crop_corner(bool dbl, const void* x, void* y, const ToepArgs& a, double sc, hipStream_t s);

@@ -411,6 +411,9 @@ struct nfftm_plan_s {
   DevBuf<int4> d_items;
   DevBuf<int> d_tstart;
   DevBuf<char> d_scratch, d_rec;
+  // node-side staging vector (M values of the plan's complex type) of the
+  // typed stage operators, allocated at their first call
+  DevBuf<char> d_tmpM;
   // NFFTM_FLAG_SPARSE: rows of W^D grid indices and window weights per node
   bool sparse = false;
   int sparseK = 0;
@@ -949,6 +952,54 @@ int run_deconv(nfftm_plan_s& p, bool fwd, const void* in, void* out, bool standa
   ProfScope ps(p, fwd ? 0 : 6);
   if (fwd) HIPCHK(launch_deconv(p.dbl(), a, p.stream, in, out));
   else HIPCHK(launch_deconv_t(p.dbl(), a, p.stream, in, out));
+  return NFFTM_OK;
+}
+
+// convolve! / convolve_transpose! on caller buffers of any element type
+// (src/convolution.jl:20-53, :115-149, which take RealOrComplex arrays of any
+// precision; test/convolve.jl:43-146).  The sums run in the plan's complex
+// type: inputs of another type are converted into the plan's grid (grid side)
+// or its node-side staging vector, outputs are rounded from there.  A real
+// input has a real result (the window is real), so a real output takes the
+// real part exactly.  Checked as the reference checks: complex input with
+// real output first (ArgumentError), then the sizes (DimensionMismatch).
+int typed_conv(nfftm_plan_s& p, bool spread, const void* in, int tin, i64 nin, void* out, int tout, i64 nout) {
+  auto valid = [](int t) { return t >= NFFTM_R32 && t <= NFFTM_C16; };
+  auto cpx = [](int t) { return t == NFFTM_C32 || t == NFFTM_C64 || t == NFFTM_C16; };
+  if (!valid(tin) || !valid(tout)) return fail(NFFTM_ERR_ARG, "element type codes %d, %d not supported", tin, tout);
+  if (cpx(tin) && !cpx(tout))
+    return fail(NFFTM_ERR_ARG, spread ? "Complex input fHat requires Complex output g"
+                                      : "Complex input g requires Complex output fHat");
+  const i64 want_in = spread ? p.M : p.Q, want_out = spread ? p.Q : p.M;
+  if (spread) {
+    if (nout != p.Q) return fail(NFFTM_ERR_DIM, "length(g) = %lld != prod(Ntilde) = %lld", (long long)nout, (long long)p.Q);
+    if (nin != p.M) return fail(NFFTM_ERR_DIM, "length(fHat) = %lld != J = %lld", (long long)nin, (long long)p.M);
+  } else {
+    if (nin != p.Q) return fail(NFFTM_ERR_DIM, "length(g) = %lld != prod(Ntilde) = %lld", (long long)nin, (long long)p.Q);
+    if (nout != p.M) return fail(NFFTM_ERR_DIM, "length(fHat) = %lld != J = %lld", (long long)nout, (long long)p.M);
+  }
+  if ((want_in && !in) || (want_out && !out)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
+  if (!spread && p.M == 0) return NFFTM_OK;
+  const int native = p.dbl() ? NFFTM_C64 : NFFTM_C32;
+  auto node_side = [&]() -> void* {
+    const size_t bytes = (size_t)std::max<i64>(1, p.M) * p.celem();
+    if (p.d_tmpM.n < bytes && p.d_tmpM.alloc(bytes) != hipSuccess) return nullptr;
+    return p.d_tmpM.p;
+  };
+  const void* src = in;
+  void* dst = out;
+  if (tin != native) {
+    void* st = spread ? node_side() : (void*)p.d_grid.p;
+    if (!st) return fail(NFFTM_ERR_OOM, "staging vector of %lld values", (long long)p.M);
+    HIPCHK(launch_convert(tin, in, native, st, want_in, p.stream));
+    src = st;
+  }
+  if (tout != native) {
+    dst = spread ? (void*)p.d_grid.p : node_side();
+    if (!dst) return fail(NFFTM_ERR_OOM, "staging vector of %lld values", (long long)p.M);
+  }
+  if (int rc = run_conv(p, spread, src, dst, true)) return rc;
+  if (tout != native) HIPCHK(launch_convert(native, dst, tout, out, want_out, p.stream));
   return NFFTM_OK;
 }
 
@@ -1648,6 +1699,20 @@ int nfftm_convolve_transpose(nfftm_plan p, const void* fhat, void* g) {
   g_err.clear();
   if (!p || !g || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL argument");
   return run_conv(*p, true, fhat, g, true);
+}
+
+int nfftm_convolve_typed(nfftm_plan p, const void* g, int g_type, int64_t g_len, void* fhat, int fhat_type,
+                         int64_t fhat_len) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  return typed_conv(*p, false, g, g_type, g_len, fhat, fhat_type, fhat_len);
+}
+
+int nfftm_convolve_transpose_typed(nfftm_plan p, const void* fhat, int fhat_type, int64_t fhat_len, void* g,
+                                   int g_type, int64_t g_len) {
+  g_err.clear();
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  return typed_conv(*p, true, fhat, fhat_type, fhat_len, g, g_type, g_len);
 }
 
 int nfftm_fft(nfftm_plan p, void* g, int direction) {

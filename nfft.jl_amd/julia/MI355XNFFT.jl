@@ -6,7 +6,7 @@ one MI355X.  It plugs into AbstractNFFTs exactly where NFFT.jl's own GPU
 extension does (ext/NFFTGPUArraysExt/implementation.jl:21-59) and exposes the
 same operator surface: plan_nfft, mul!, adjoint, nodes!, size_in/size_out,
 convolve!/convolve_transpose!/deconvolve!/deconvolve_transpose!, copy and
-TimingStats.  Data buffers are AMDGPU `ROCArray`s passed as device pointers;
+TimingStats, and NFFTTools' sdc! and Toeplitz kernel on the device.  Data buffers are AMDGPU `ROCArray`s passed as device pointers;
 node matrices stay on the host, as `k::Matrix` does in the reference.
 
 Usage:
@@ -27,6 +27,7 @@ using AbstractNFFTs: AbstractNFFTBackend, TimingStats
 using LinearAlgebra
 using AMDGPU
 import ChainRulesCore
+import NFFTTools
 
 const libnfftm = get(ENV, "NFFTM_LIB", joinpath(@__DIR__, "..", "nfftm", "libnfftm.so"))
 
@@ -180,15 +181,121 @@ function LinearAlgebra.mul!(f::ROCArray, pl::Adjoint{Complex{T},<:MI355XNFFTPlan
   return f
 end
 
-# Stage operators used by NFFTTools (samplingDensity.jl:106,112):
-# src/convolution.jl:20-53, :115-149; src/deconvolution.jl:2-92.
-for (jl, c) in ((:convolve!, :nfftm_convolve), (:convolve_transpose!, :nfftm_convolve_transpose),
-                (:deconvolve!, :nfftm_deconvolve), (:deconvolve_transpose!, :nfftm_deconvolve_transpose))
-  @eval function AbstractNFFTs.$jl(p::MI355XNFFTPlan, a::ROCArray, b::ROCArray)
-    check(ccall(($(QuoteNode(c)), libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
-                p.handle, devptr(a), devptr(b)))
-    return b
+# Element types of the typed stage operators (include/nfftm.h NFFTM_R32 ...).
+const ELT = Dict{DataType,Cint}(Float32 => Cint(0), Float64 => Cint(1), ComplexF32 => Cint(2),
+                                ComplexF64 => Cint(3), Float16 => Cint(4), ComplexF16 => Cint(5))
+elt(::Type{E}) where {E} = haskey(ELT, E) ? ELT[E] :
+  throw(ArgumentError("element type $(E) is not supported by the MI355X stage operators"))
+
+# convolve!(p, g, fHat): src/convolution.jl:20-53.  g and fHat may be real or
+# complex of any precision (RealOrComplex, as NFFTTools.sdc! passes real
+# weights and grids, samplingDensity.jl:106,112); the library converts
+# through the plan's complex type and checks the lengths again.
+function AbstractNFFTs.convolve!(p::MI355XNFFTPlan{T,D}, g::ROCArray{<:Any,D}, fHat::ROCVector) where {T,D}
+  (eltype(g) <: Complex && !(eltype(fHat) <: Complex)) &&
+    throw(ArgumentError("Complex input g requires Complex output fHat"))
+  size(g) == p.Ñ || throw(DimensionMismatch("size(g)=$(size(g)) ≠ Ñ = $(p.Ñ)"))
+  size(fHat) == (p.J,) || throw(DimensionMismatch("size(fHat)=$(size(fHat)) ≠ J = $(p.J)"))
+  check(ccall((:nfftm_convolve_typed, libnfftm), Cint,
+              (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}, Cint, Int64),
+              p.handle, devptr(g), elt(eltype(g)), length(g), devptr(fHat), elt(eltype(fHat)), length(fHat)))
+  return fHat
+end
+
+# convolve_transpose!(p, fHat, g): src/convolution.jl:115-149.
+function AbstractNFFTs.convolve_transpose!(p::MI355XNFFTPlan{T,D}, fHat::ROCVector, g::ROCArray{<:Any,D}) where {T,D}
+  (eltype(fHat) <: Complex && !(eltype(g) <: Complex)) &&
+    throw(ArgumentError("Complex input fHat requires Complex output g"))
+  size(g) == p.Ñ || throw(DimensionMismatch("size(g)=$(size(g)) ≠ Ñ = $(p.Ñ)"))
+  size(fHat) == (p.J,) || throw(DimensionMismatch("size(fHat)=$(size(fHat)) ≠ J = $(p.J)"))
+  check(ccall((:nfftm_convolve_transpose_typed, libnfftm), Cint,
+              (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}, Cint, Int64),
+              p.handle, devptr(fHat), elt(eltype(fHat)), length(fHat), devptr(g), elt(eltype(g)), length(g)))
+  return g
+end
+
+# deconvolve!(p, f, g) and deconvolve_transpose!(p, g, f):
+# src/deconvolution.jl:2-92, on Complex{T} buffers of the plan's sizes.
+function AbstractNFFTs.deconvolve!(p::MI355XNFFTPlan{T,D}, f::ROCArray{Complex{T},D},
+                                   g::ROCArray{Complex{T},D}) where {T,D}
+  size(f) == p.N || throw(DimensionMismatch("size(f)=$(size(f)) ≠ N = $(p.N)"))
+  size(g) == p.Ñ || throw(DimensionMismatch("size(g)=$(size(g)) ≠ Ñ = $(p.Ñ)"))
+  check(ccall((:nfftm_deconvolve, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+              p.handle, devptr(f), devptr(g)))
+  return g
+end
+
+function AbstractNFFTs.deconvolve_transpose!(p::MI355XNFFTPlan{T,D}, g::ROCArray{Complex{T},D},
+                                             f::ROCArray{Complex{T},D}) where {T,D}
+  size(f) == p.N || throw(DimensionMismatch("size(f)=$(size(f)) ≠ N = $(p.N)"))
+  size(g) == p.Ñ || throw(DimensionMismatch("size(g)=$(size(g)) ≠ Ñ = $(p.Ñ)"))
+  check(ccall((:nfftm_deconvolve_transpose, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+              p.handle, devptr(g), devptr(f)))
+  return f
+end
+
+# NFFTTools on the plan.  sdc!(p, iters, weights, weights_tmp, workg[, workf,
+# workv]) (NFFTTools/src/samplingDensity.jl:93-155): the iterations run on
+# the device (nfftm_sdc); the work buffers of the generic method are not
+# needed.  Throws "non-positive weights" as the reference does.
+function NFFTTools.sdc!(p::MI355XNFFTPlan{T}, iters::Int, weights::ROCVector{T}, weights_tmp::AbstractVector,
+                        workg::AbstractArray) where {T<:Real}
+  length(weights) == p.J || throw(DimensionMismatch("length(weights)=$(length(weights)) ≠ J = $(p.J)"))
+  rc = ccall((:nfftm_sdc, libnfftm), Cint, (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Cint), p.handle, iters, devptr(weights), 0)
+  rc == NFFTM_ERR_UNSUPPORTED && throw("non-positive weights")
+  check(rc)
+  return weights
+end
+
+function NFFTTools.sdc!(p::MI355XNFFTPlan{T}, iters::Int, weights::ROCVector{T}, weights_tmp::AbstractVector,
+                        workg::AbstractArray, workf::AbstractVector, workv::AbstractArray) where {T<:Real}
+  length(weights) == p.J || throw(DimensionMismatch("length(weights)=$(length(weights)) ≠ J = $(p.J)"))
+  rc = ccall((:nfftm_sdc, libnfftm), Cint, (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Cint), p.handle, iters, devptr(weights), 1)
+  rc == NFFTM_ERR_UNSUPPORTED && throw("non-positive weights")
+  check(rc)
+  return weights
+end
+
+# Toeplitz embedding (NFFTTools/src/Toeplitz.jl): one library object per
+# (precision, image size), created on first use and freed with the process.
+mutable struct ToeplitzObj
+  handle::Ptr{Cvoid}
+end
+const TOEPLITZ = Dict{Tuple{DataType,Tuple},ToeplitzObj}()
+function toeplitz_obj(::Type{T}, N::NTuple{D,Int}) where {T,D}
+  get!(TOEPLITZ, (T, N)) do
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:nfftm_toeplitz_create, libnfftm), Cint, (Ref{Ptr{Cvoid}}, Cint, Cint, Ptr{Int64}, Ptr{Cvoid}),
+                h, precision_of(T), D, collect(Int64, N), AMDGPU.stream().stream))
+    o = ToeplitzObj(h[])
+    finalizer(x -> ccall((:nfftm_toeplitz_destroy, libnfftm), Cvoid, (Ptr{Cvoid},), x.handle), o)
+    o
   end
+end
+
+# calculateToeplitzKernel!(f, p, tr, fftplan) (Toeplitz.jl:119-125): nodes!(p,
+# tr), then FFT(fftshift(adjoint(p) * ones)) on the device into f (size 2N =
+# the plan's N); fftplan is not used.
+function NFFTTools.calculateToeplitzKernel!(f::ROCArray{Complex{T},D}, p::MI355XNFFTPlan{T,D}, tr::Matrix{T},
+                                            fftplan=nothing) where {T,D}
+  size(f) == p.N || throw(DimensionMismatch("size(f)=$(size(f)) ≠ the plan's N = $(p.N)"))
+  all(iseven, p.N) || throw(ArgumentError("the Toeplitz kernel needs a plan of size 2N"))
+  AbstractNFFTs.nodes!(p, tr)
+  t = toeplitz_obj(T, p.N .÷ 2)
+  check(ccall((:nfftm_toeplitz_kernel, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+              t.handle, p.handle, devptr(f)))
+  return f
+end
+
+# convolveToeplitzKernel!(y, λ, ...) (Toeplitz.jl:236-250): y overwritten by
+# the N-corner of IFFT(FFT(pad(y)) .* λ); the optional plans and buffers of
+# the generic method are not needed.
+function NFFTTools.convolveToeplitzKernel!(y::ROCArray{Complex{T},D}, λ::ROCArray{Complex{T},D}, args...) where {T,D}
+  size(λ) == 2 .* size(y) || throw(DimensionMismatch("size(λ)=$(size(λ)) ≠ 2 .* size(y) = $(2 .* size(y))"))
+  t = toeplitz_obj(T, size(y))
+  check(ccall((:nfftm_toeplitz_apply, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+              t.handle, devptr(λ), devptr(y)))
+  return y
 end
 
 """

@@ -472,13 +472,45 @@ class NFFTPlan:
             out.copy_(o.real if real_out else o)
         return out
 
+    _ELT = {torch.float32: _L.R32, torch.float64: _L.R64, torch.complex64: _L.C32,
+            torch.complex128: _L.C64, torch.float16: _L.R16, torch.complex32: _L.C16}
+
+    def _typed(self, fn, inp, in_shape, out, out_shape, names):
+        """Stage operator on tensors of any real or complex type (the
+        reference's RealOrComplex methods), through the typed C entry point:
+        the library converts to and from the plan's complex type."""
+        if isinstance(inp, np.ndarray):
+            inp = torch.from_numpy(np.ascontiguousarray(inp))
+        if not isinstance(inp, torch.Tensor):
+            inp = torch.as_tensor(inp)
+        if not isinstance(out, torch.Tensor):
+            raise ArgumentError(f"{names[1]} must be a device tensor")
+        if inp.dtype not in self._ELT or out.dtype not in self._ELT:
+            raise ArgumentError(f"element types {inp.dtype}, {out.dtype} not supported")
+        if inp.is_complex() and not out.is_complex():
+            raise ArgumentError(f"Complex input {names[0]} requires Complex output {names[1]}")
+        if tuple(inp.shape) != tuple(in_shape):
+            raise DimensionMismatch(f"size({names[0]})={tuple(inp.shape)} does not match {tuple(in_shape)}")
+        if tuple(out.shape) != tuple(out_shape):
+            raise DimensionMismatch(f"size({names[1]})={tuple(out.shape)} does not match {tuple(out_shape)}")
+        x = inp.to(device=_device()).contiguous()
+        o = out if (out.is_contiguous() and out.is_cuda) else torch.empty_like(out, device=_device())
+        self._lib.nfftm_set_stream(self._handle, _stream())
+        _check(fn(self._handle, ctypes.c_void_p(x.data_ptr()), self._ELT[x.dtype], x.numel(),
+                  ctypes.c_void_p(o.data_ptr()), self._ELT[o.dtype], o.numel()))
+        if o is not out:
+            out.copy_(o)
+        return out
+
     def convolve_(self, g, fhat):
-        """convolve!(p, g, fHat), src/convolution.jl:20-53."""
-        return self._stage(self._lib.nfftm_convolve, g, self.Ntilde, fhat, (self.J,), ("g", "fHat"))
+        """convolve!(p, g, fHat), src/convolution.jl:20-53 (g and fHat real or
+        complex, any precision)."""
+        return self._typed(self._lib.nfftm_convolve_typed, g, self.Ntilde, fhat, (self.J,), ("g", "fHat"))
 
     def convolve_transpose_(self, fhat, g):
-        """convolve_transpose!(p, fHat, g), src/convolution.jl:115-149."""
-        return self._stage(self._lib.nfftm_convolve_transpose, fhat, (self.J,), g, self.Ntilde,
+        """convolve_transpose!(p, fHat, g), src/convolution.jl:115-149 (fHat and
+        g real or complex, any precision)."""
+        return self._typed(self._lib.nfftm_convolve_transpose_typed, fhat, (self.J,), g, self.Ntilde,
                            ("fHat", "g"))
 
     def deconvolve_(self, f, g):

@@ -17,12 +17,13 @@ FULL, TENSOR, LINEAR, POLYNOMIAL = 1, 2, 3, 4
 KAISER_BESSEL = 0
 FLAG_BLOCKING, FLAG_TIMING, FLAG_GRAPHS, FLAG_ROCFFT, FLAG_SEPARATE_FOLD, FLAG_SPARSE = 1, 2, 4, 8, 16, 32
 BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
+R32, R64, C32, C64, R16, C16 = range(6)  # element types of the typed stage operators
 
 # every symbol include/nfftm.h declares
 EXPORTS = (
     "nfftm_plan_create", "nfftm_nodes", "nfftm_forward", "nfftm_adjoint",
     "nfftm_deconvolve", "nfftm_deconvolve_transpose", "nfftm_convolve",
-    "nfftm_convolve_transpose", "nfftm_fft", "nfftm_sizes", "nfftm_grid",
+    "nfftm_convolve_transpose", "nfftm_convolve_typed", "nfftm_convolve_transpose_typed", "nfftm_fft", "nfftm_sizes", "nfftm_grid",
     "nfftm_timing", "nfftm_set_timing", "nfftm_profile", "nfftm_profile_read",
     "nfftm_set_stream", "nfftm_plan_copy", "nfftm_grad_nodes", "nfftm_plan_tune",
     "nfftm_tile_shape", "nfftm_plan_set_tiles", "nfftm_batch_create", "nfftm_batch_info", "nfftm_batch_forward",
@@ -71,6 +72,8 @@ def load():
     for name in ("nfftm_forward", "nfftm_adjoint", "nfftm_deconvolve",
                  "nfftm_deconvolve_transpose", "nfftm_convolve", "nfftm_convolve_transpose"):
         getattr(L, name).argtypes = [vp, vp, vp]
+    L.nfftm_convolve_typed.argtypes = [vp, vp, i, i64, vp, i, i64]
+    L.nfftm_convolve_transpose_typed.argtypes = [vp, vp, i, i64, vp, i, i64]
     L.nfftm_fft.argtypes = [vp, vp, i]
     L.nfftm_sizes.argtypes = [vp, pi64, pi64, pd]
     L.nfftm_grid.argtypes = [vp, ctypes.POINTER(vp)]

@@ -88,6 +88,7 @@ def lib():
         L.orc_window_du.argtypes = [D, I, D]
         L.orc_grad_gather.argtypes = [P, dp, dp, dp]
         L.orc_ndft_grad.argtypes = [I, lp, LL, dp, dp, dp, dp, lp, LL]
+        L.orc_ndft_adjoint_box.argtypes = [I, lp, LL, dp, dp, dp, lp, lp]
         _lib = L
     return _lib
 
@@ -198,6 +199,24 @@ def ndft_adjoint(k, N, fhat, sel=None):
     lib().orc_ndft_adjoint(D, _lp(Na), M, _dp(kj), _dp(fhat.view(np.float64)),
                            _dp(out.view(np.float64)), _lp(sel), len(sel))
     return out
+
+
+def ndft_adjoint_box(k, N, fhat, idx):
+    """Adjoint NDFT on the product set idx[0] x ... x idx[D-1] of image
+    indices (Python axis order, one index array per axis); returns an array
+    of shape (len(idx[0]), ..., len(idx[D-1])).  Same sums as
+    ``ndft_adjoint``, with the phase factorised per node."""
+    kj, D, M = _julia_nodes(k)
+    assert len(idx) == D == len(N)
+    fhat = _c128(fhat)
+    Na = np.array(tuple(N)[::-1], dtype=np.int64)
+    lists = [np.asarray(i, dtype=np.int64) for i in idx[::-1]]  # Julia order
+    flat = np.ascontiguousarray(np.concatenate(lists))
+    cnt = np.array([len(i) for i in lists], dtype=np.int64)
+    out = np.empty(int(np.prod(cnt)), dtype=np.complex128)
+    lib().orc_ndft_adjoint_box(D, _lp(Na), M, _dp(kj), _dp(fhat.view(np.float64)),
+                               _dp(out.view(np.float64)), _lp(flat), _lp(cnt))
+    return out.reshape(tuple(len(i) for i in idx))
 
 
 def ndft_grad(k, f, ybar, sel=None):

@@ -1114,4 +1114,60 @@ void orc_ndft_grad(int D, const long long* N, long long M, const double* k, cons
   }
 }
 
+// Direct adjoint NDFT, src/direct.jl:126-151, on a product set of grid
+// points: along dimension d the points are idx[off_d + 0 .. cnt[d]-1]
+// (0-based image indices i_d, frequency n_d = i_d - floor(N_d/2)), off_d =
+// cnt[0] + ... + cnt[d-1].  f has prod(cnt) entries, dimension 0 fastest.
+// Per node the phase factorises, exp(+2 pi i n.k_j) = prod_d exp(2 pi i
+// n_d k_dj), so each node costs sum(cnt) sin/cos pairs (exact phase
+// reduction as above) plus prod(cnt) complex products, instead of prod(cnt)
+// D-term phases.  Nodes are split over threads; each thread sums into its
+// own output, and the partial outputs are added at the end.
+void orc_ndft_adjoint_box(int D, const long long* N, long long M, const double* k, const double* fhat_,
+                          double* f_, const long long* idx, const long long* cnt) {
+  const cd* fhat = (const cd*)fhat_;
+  cd* f = (cd*)f_;
+  i64 nout = 1, tot = 0;
+  for (int d = 0; d < D; ++d) {
+    nout *= cnt[d];
+    tot += cnt[d];
+  }
+  std::fill(f, f + nout, cd(0, 0));
+#pragma omp parallel
+  {
+    std::vector<cd> mine(nout, cd(0, 0)), ph(tot), part(nout);
+#pragma omp for schedule(static)
+    for (i64 j = 0; j < M; ++j) {
+      i64 o = 0;
+      for (int d = 0; d < D; ++d) {
+        const double kd = k[d + (i64)D * j];
+        for (i64 q = 0; q < cnt[d]; ++q) {
+          const double a = 2.0 * M_PI * frac_prod(kd, (double)(idx[o + q] - N[d] / 2));
+          ph[o + q] = cd(std::cos(a), std::sin(a));
+        }
+        o += cnt[d];
+      }
+      // outer product of the per-dimension phases, built from the last
+      // dimension (slowest) to dimension 0 (fastest); walking r downwards
+      // never overwrites an entry before it is read
+      i64 len = 1;
+      part[0] = fhat[j];
+      o = tot;
+      for (int d = D - 1; d >= 0; --d) {
+        o -= cnt[d];
+        for (i64 r = len - 1; r >= 0; --r) {
+          const cd v = part[r];
+          for (i64 q = cnt[d] - 1; q >= 0; --q) part[r * cnt[d] + q] = v * ph[o + q];
+        }
+        len *= cnt[d];
+      }
+      for (i64 t = 0; t < nout; ++t) mine[t] += part[t];
+    }
+#pragma omp critical
+    {
+      for (i64 t = 0; t < nout; ++t) f[t] += mine[t];
+    }
+  }
+}
+
 }  // extern "C"

@@ -161,3 +161,107 @@ def test_flag_values_agree_across_bindings():
     jl = open(os.path.join(REPO, "nfft.jl_amd", "julia", "MI355XNFFT.jl")).read()
     jlf = {k: int(v) for k, v in re.findall(r"const NFFTM_FLAG_([A-Z_]+) = Cint\((\d+)\)", jl)}
     assert jlf == hdr
+
+
+def _split_top(s, sep=","):
+    """Split s at separators outside (), {} and []."""
+    out, depth, cur = [], 0, ""
+    for ch in s:
+        if ch in "({[":
+            depth += 1
+        elif ch in ")}]":
+            depth -= 1
+        if ch == sep and depth == 0:
+            out.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        out.append(cur)
+    return [x.strip() for x in out]
+
+
+def header_prototypes():
+    """{name: (return type, [parameter types])} of include/nfftm.h, types
+    normalised ('const' dropped, arrays as pointers)."""
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(REPO, "include", "nfftm.h")).read(), flags=re.S)
+    protos = {}
+    for ret, name, params in re.findall(r"([A-Za-z_][\w\s\*]*?)\b(nfftm_\w+)\s*\(([^)]*)\)\s*;", txt):
+        ret = ret.split()[-1] if "*" not in ret else ret
+        types = []
+        for prm in _split_top(params):
+            if prm in ("", "void"):
+                continue
+            arr = "[" in prm
+            prm = re.sub(r"\[.*?\]", "", prm).replace("const", " ")
+            toks = prm.replace("*", " * ").split()
+            base = [t for t in toks[:-1]]  # the last token is the parameter name
+            ty = " ".join(t for t in base if t != "*") + "*" * base.count("*") + ("*" if arr else "")
+            types.append(ty.replace(" *", "*"))
+        protos[name] = (" ".join(ret.replace("const", " ").split()).replace(" *", "*"), types)
+    return protos
+
+
+# C parameter type -> Julia ccall argument types that pass it correctly
+_JL_OK = {
+    "int": {"Cint"}, "int64_t": {"Int64"}, "double": {"Cdouble", "Float64"},
+    "void*": {"Ptr{Cvoid}"}, "double*": {"Ptr{Float64}", "Ref{Float64}", "Ptr{Cvoid}"},
+    "int64_t*": {"Ptr{Int64}", "Ref{Int64}"}, "int*": {"Ptr{Cint}", "Ref{Cint}"},
+    "void**": {"Ref{Ptr{Cvoid}}", "Ptr{Ptr{Cvoid}}"}, "void* *": {"Ptr{Ptr{Cvoid}}"},
+    "nfftm_plan": {"Ptr{Cvoid}"}, "nfftm_batch": {"Ptr{Cvoid}"}, "nfftm_toeplitz": {"Ptr{Cvoid}"},
+    "nfftm_plan*": {"Ref{Ptr{Cvoid}}", "Ptr{Ptr{Cvoid}}"}, "nfftm_batch*": {"Ref{Ptr{Cvoid}}"},
+    "nfftm_toeplitz*": {"Ref{Ptr{Cvoid}}"},
+}
+_JL_RET = {"int": "Cint", "void": "Cvoid", "char*": "Cstring"}
+
+
+def julia_ccalls():
+    """[(name, return type, [argument types])] of every ccall in the shim."""
+    src = open(os.path.join(REPO, "nfft.jl_amd", "julia", "MI355XNFFT.jl")).read()
+    out = []
+    for mt in re.finditer(r"ccall\(\(:(\w+),\s*libnfftm\),", src):
+        depth, j = 0, mt.end()
+        while depth > 0 or src[j] != ")":  # up to the ccall's own closing parenthesis
+            depth += src[j] in "({[" and 1 or src[j] in ")}]" and -1 or 0
+            j += 1
+        parts = _split_top(src[mt.end():j])
+        ret, tup = parts[0], parts[1]
+        assert tup.startswith("(") and tup.endswith(")"), (mt.group(1), tup)
+        out.append((mt.group(1), ret, _split_top(tup[1:-1])))
+    return out
+
+
+def test_julia_ccalls_match_header():
+    """Every ccall of the Julia binding names an exported entry point with the
+    header's arity, argument types and return type."""
+    protos = header_prototypes()
+    assert "nfftm_convolve_typed" in protos and len(protos) == len(header_symbols())
+    calls = julia_ccalls()
+    assert len(calls) >= 20
+    seen = set()
+    for name, ret, args in calls:
+        assert name in protos, name
+        cret, cargs = protos[name]
+        assert _JL_RET[cret] == ret, (name, ret, cret)
+        assert len(args) == len(cargs), (name, args, cargs)
+        for a, c in zip(args, cargs):
+            assert a in _JL_OK[c], (name, a, c)
+        seen.add(name)
+    # the stage operators, sdc and Toeplitz are reached from Julia
+    for need in ("nfftm_convolve_typed", "nfftm_convolve_transpose_typed", "nfftm_deconvolve",
+                 "nfftm_deconvolve_transpose", "nfftm_sdc", "nfftm_toeplitz_create", "nfftm_toeplitz_kernel",
+                 "nfftm_toeplitz_apply", "nfftm_toeplitz_destroy"):
+        assert need in seen, need
+
+
+def test_element_type_codes_agree():
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(REPO, "include", "nfftm.h")).read(), flags=re.S)
+    hdr = {k: int(v) for k, v in re.findall(r"\bNFFTM_([RC]\d\d)\s*=\s*(\d+)", txt)}
+    assert hdr == {"R32": 0, "R64": 1, "C32": 2, "C64": 3, "R16": 4, "C16": 5}
+    for k, v in hdr.items():
+        assert getattr(_lib, k) == v
+    jl = open(os.path.join(REPO, "nfft.jl_amd", "julia", "MI355XNFFT.jl")).read()
+    jmap = dict(re.findall(r"(\w+) => Cint\((\d)\)", jl[jl.index("const ELT"):jl.index("elt(::Type")]))
+    names = {"Float32": "R32", "Float64": "R64", "ComplexF32": "C32", "ComplexF64": "C64", "Float16": "R16",
+             "ComplexF16": "C16"}
+    assert {names[k]: int(v) for k, v in jmap.items()} == hdr

@@ -53,6 +53,28 @@ def test_ndft_vs_mpmath():
         assert O.relerr(O.ndft_adjoint(k, N, fh), ad) < 1e-14
 
 
+@pytest.mark.parametrize("N", [(9,), (6, 7), (5, 4, 6)])
+def test_ndft_adjoint_box_equals_pointwise(N):
+    """The factorised product-set adjoint NDFT equals the pointwise adjoint
+    NDFT (itself pinned by the mpmath fixtures above) on every point of the
+    set, including repeated and unsorted indices."""
+    rng = np.random.default_rng(7)
+    D = len(N)
+    M = 300
+    k = rng.random((D, M)) - 0.5
+    fh = crandn(rng, M)
+    idx = [rng.choice(n, size=min(n, 3 + d), replace=False) for d, n in enumerate(N)]
+    idx[0] = np.concatenate([idx[0], idx[0][:1]])
+    got = O.ndft_adjoint_box(k, N, fh, idx)
+    assert got.shape == tuple(len(i) for i in idx)
+    mesh = np.meshgrid(*idx, indexing="ij")
+    flat = np.ravel_multi_index([g.ravel() for g in mesh], N)
+    ref = O.ndft_adjoint(k, N, fh, flat).reshape(got.shape)
+    assert O.relerr(got, ref) < 1e-13
+    full = O.ndft_adjoint(k, N, fh)
+    assert O.relerr(got, full[np.ix_(*idx)]) < 1e-13
+
+
 @pytest.mark.parametrize("N", [(16,), (7,), (6, 5), (4, 3, 5)])
 def test_ndft_equispaced_is_shifted_dft(N):
     """On the grid k = i/N - 1/2 (per axis) the NDFT is an FFT with shifts."""
