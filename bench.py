@@ -41,25 +41,63 @@ sys.path[:0] = [REPO, os.path.join(REPO, "nfft.jl_amd")]
 import numpy as np  # noqa: E402
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
+F32_TOL = 1e-5  # rel-l2 bar of every FP32 result (against the FP64 plan and the FP64 NDFT)
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (scaled by a 256 MiB elementwise copy in the same run: factors 2.0 / 1.0);
 # profiles/nfft3d_128cubed_M2p21_hbm_traffic.md holds the passes and the
 # command.  Keyed by (workload, stage) for the default FP32 plan of that
-# workload (3-D: k_spread3<float, 8, 1024, 1>, flat 16 x 16 x 8 tiles).
-MEASURED_TRAFFIC = {("3d", "spreading"): 1075470013, ("3d", "interpolation"): 564103562}
+# workload, together with the digest of the kernel sources they were
+# measured on (source_digest()): when the sources change, the figure is
+# reported as stale (traffic null) until it is measured again.
+MEASURED_TRAFFIC = {("3d", "spreading"): (1075470013, None), ("3d", "interpolation"): (564103562, None)}
 
-STAGES = ("deconvolve", "fft_forward", "interpolation", "grid_clear", "spreading", "fft_backward",
+# Stage order of nfftm_profile_read.  "fold": the per-item padded tiles summed
+# into the grid; on 3-D power-of-two grids with one work item per tile it is
+# fused with the first adjoint FFT pass (lines along dimension 3).
+STAGES = ("deconvolve", "fft_forward", "interpolation", "fold", "spreading", "fft_backward",
           "deconvolve_transpose", "node_gradient")
+
+# Measured issue cost of the LDS operations of the gridding taps on MI355X
+# (cycles per wave64 instruction per CU, microbenchmark in the library:
+# nfftm_lds_issue_cycles), and the clock they are counted at.
+CLOCK_HZ = 2.4e9
+N_CU = 256
+
+
+def source_digest():
+    """sha256 (16 hex digits) of the HIP sources the kernels are built from."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(REPO, "nfft.jl_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".hpp", ".inc")):
+            with open(os.path.join(d, name), "rb") as fh:
+                h.update(name.encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def stage_bytes(D, P, Q, M, sc, sk=8):
-    """Algorithmic HBM bytes per launch of each stage."""
+    """Algorithmic HBM bytes per launch of each stage.  The fold is credited
+    with the grid it produces (Q s_c); in 3-D it is fused with the first
+    adjoint FFT pass (lines along dimension 3), whose algorithmic bytes are
+    one grid read and the half of the grid the next pass reads: 1.5 Q s_c."""
     return {
         "deconvolve": P * sc + Q * sc,
         "fft_forward": 2 * Q * sc,
         "interpolation": M * (D * sk + sc) + Q * sc,
-        "grid_clear": Q * sc,
+        "fold": (1.5 if D == 3 else 1.0) * Q * sc,
         "spreading": M * (D * sk + sc) + Q * sc,
         "fft_backward": 2 * Q * sc,
         "deconvolve_transpose": 2 * P * sc,
@@ -369,7 +407,19 @@ def main():
     fh_h = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
 
     log("bench: creating FP32 and FP64 plans")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     p32 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float32)
+    torch.cuda.synchronize()
+    pre32 = time.perf_counter() - t0
+    # nodes! on the same nodes (device binning, records): the per-update cost
+    # of a trajectory that changes between transforms
+    t_nodes = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        p32.nodes_(k)
+        torch.cuda.synchronize()
+        t_nodes = min(t_nodes, time.perf_counter() - t0)
     if args.tune:
         log(f"bench: tuned FP32 tiles {p32.tune()}")
     Q = int(np.prod(p32.Ntilde))
@@ -409,7 +459,11 @@ def main():
     stage_ms = {s: (ms[i] / cnt[i] if cnt[i] else None) for i, s in enumerate(STAGES)}
 
     # FP64 plan on the same workload
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     p64 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float64)
+    torch.cuda.synchronize()
+    pre64 = time.perf_counter() - t0
     if args.tune:
         log(f"bench: tuned FP64 tiles {p64.tune()}")
     f64 = f32.to(torch.complex128)
@@ -501,19 +555,23 @@ def main():
     err["adj_f32_vs_ndft"] = float(np.linalg.norm(x32n.ravel()[gsel] - ref) / np.linalg.norm(ref))
     err["adj_f64_vs_ndft"] = float(np.linalg.norm(x64n.ravel()[gsel] - ref) / np.linalg.norm(ref))
     err["samples"] = {"fwd_nodes": nfwd, "adj_grid_points": nadj}
+    accuracy_ok = all(v <= F32_TOL for key, v in err.items() if "f32" in key)
 
     # roofline of the dominant hand-written kernel
     sc = 8
     sb = stage_bytes(D, P, Q, M, sc)
-    mine = ("deconvolve", "interpolation", "spreading", "deconvolve_transpose", "grid_clear",
+    mine = ("deconvolve", "interpolation", "spreading", "deconvolve_transpose", "fold",
             "node_gradient")
     dom = max((s for s in mine if stage_ms[s]), key=lambda s: stage_ms[s])
     achieved = sb[dom] / (stage_ms[dom] * 1e-3)
-    traffic = None if args.tune else MEASURED_TRAFFIC.get((args.workload, dom))
+    digest = source_digest()
+    meas = None if args.tune else MEASURED_TRAFFIC.get((args.workload, dom))
+    traffic = meas[0] if meas is not None and meas[1] == digest else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "bytes_per_launch": sb[dom], "avg_ms": stage_ms[dom],
-                "traffic_over_algorithmic": None if traffic is None else traffic / sb[dom]}
+                "traffic_over_algorithmic": None if traffic is None else traffic / sb[dom],
+                "traffic_source_digest": None if meas is None else meas[1], "source_digest": digest}
     B_total = (P * sc + 4 * Q * sc + M * (D * 8 + sc)) + (2 * P * sc + 3 * Q * sc + M * (D * 8 + sc))
 
     cpu = None
@@ -535,6 +593,7 @@ def main():
             best = min(best, time.perf_counter() - t0)
             reps += 1
         cpu = {"value": M / best, "unit": "NU points/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(),
                "sample": f"full workload ({desc}; FP64 blocked {pre_name} plan), min of {reps} "
                          "forward+adjoint runs, FFT by scipy.fft"}
 
@@ -557,6 +616,8 @@ def main():
         "fp64": {"value": M / t64, "ms_per_step": t64 * 1e3},
         "f32_over_f64": t64 / t32,
         "errors": err,
+        "accuracy_ok": accuracy_ok,
+        "plan_s": {"f32": pre32, "f64": pre64, "nodes_f32": t_nodes},
         "stage_ms": stage_ms,
         "tiles": {"f32": list(p32.tile_shape()), "f64": list(p64.tile_shape())},
         "roofline": roofline,
@@ -565,6 +626,9 @@ def main():
     if sparse is not None:
         out["sparse_matrix_baseline"] = sparse
     print(json.dumps(out))
+    if not accuracy_ok:
+        log(f"bench: FP32 error above {F32_TOL:g}: {err}")
+        sys.exit(3)
     if args.copy_calib:
         # a device-to-device copy of exactly 256 MiB read + 256 MiB written:
         # the counters it produces calibrate FETCH_SIZE / WRITE_SIZE per byte

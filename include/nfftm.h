@@ -197,6 +197,18 @@ int nfftm_set_timing(nfftm_plan p, int on);
 int nfftm_profile(nfftm_plan p, int on);
 int nfftm_profile_read(nfftm_plan p, double ms[8], int64_t count[8]);
 
+/* The node binning of the last nodes! (or plan creation): the number of
+ * work items; with non-NULL buffers also perm_host[s] (M entries: caller
+ * index of the node at sorted position s) and items_host (4 per work item:
+ * tile index, first and one-past-last sorted position, 0).  Tiles are
+ * numbered with dimension 1 fastest; a tile's nodes keep the caller's order;
+ * each work item holds at most 1024 (D = 3), 2048 (D = 2) or 4096 (D = 1)
+ * nodes of one tile.  Plans on the node-parallel path have no items and the
+ * identity permutation.  The reference's counterpart is the block layout of
+ * _precomputeBlocks (src/precomputation.jl:459-520); exposed for tests and
+ * tools.  Synchronises the plan's stream. */
+int nfftm_binning(nfftm_plan p, int* nitems, int32_t* perm_host, int32_t* items_host);
+
 /* Autotuner: times the plan's gridding kernels (spreading with its fold,
  * plus interpolation) on its own nodes for each candidate tile edge (D = 1:
  * 512..8192, D = 2: 16..128) and workgroup size (256, 512; D = 3: 512, 1024)

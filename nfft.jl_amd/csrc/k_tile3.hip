@@ -314,13 +314,21 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
     R* red = (R*)lrec;
     if (lane == 63) red[wave] = part;
     __syncthreads();
-    R B = 0;
-    for (int w = 0; w < NW; ++w) B += red[w];
-    B = (R)((double)B * a.oscale * a.wmaxD * 1.0000001);
+    double B = 0;
+    for (int w = 0; w < NW; ++w) B += (double)red[w];
+    B *= a.oscale * a.wmaxD * 1.0000001;
     int e = 0;
-    if (B > 0) (void)frexp((double)B, &e);
+    if (B > 0) (void)frexp(B, &e);
+    // both powers stay normal floats: below 2^-96 the quantum stops at 2^-126
+    // (values under ~1e-38 round to 0); a non-finite bound (Inf or NaN data)
+    // spreads zeros and turns the whole tile into NaN at the flush
+    e = max(e, -96);
     fscale = (R)ldexp(1.0, 30 - e);
     inv = (R)ldexp(1.0, e - 30);
+    if (!(B <= 1.0e300)) {
+      fscale = 0;
+      inv = (R)NAN;
+    }
   }
   for (int i = threadIdx.x; i < (int)(G::TILE_BYTES / 16); i += NT) ((uint4*)tile)[i] = make_uint4(0, 0, 0, 0);
 

@@ -110,6 +110,25 @@ hipError_t launch_deconv_batch(bool dbl, int nblk, hipStream_t s, const DArgs* d
 hipError_t launch_deconv_t_batch(bool dbl, int nblk, hipStream_t s, const DArgs* da, const int4* work, int nwork,
                                  const void* const* in, void* const* out);
 
+// Node binning on the device (k_bin.hip).  Tiles per dimension as the tile
+// kernels use them; CH = most nodes of one work item; key_bits covers the
+// tile indices (ntile <= 2^key_bits).
+struct BinArgs {
+  int D;
+  long long M;
+  int Nt[3], T[3], ntiles[3];
+  int ntile, CH, key_bits;
+};
+size_t bin_sort_temp_bytes(long long M, int key_bits);
+hipError_t launch_bin_keys(const BinArgs& a, const double* k, int* key, int* idx, int* cnt, hipStream_t s);
+hipError_t launch_bin_sort(const BinArgs& a, void* temp, size_t temp_bytes, int* key_in, int* key_out, int* idx_in,
+                           int* idx_out, hipStream_t s);
+hipError_t launch_bin_scan(const BinArgs& a, const int* cnt, int* off, int* ioff, int* stats, hipStream_t s);
+hipError_t launch_bin_items(const BinArgs& a, const int* off, const int* ioff, int4* items, int* tstart,
+                            hipStream_t s);
+hipError_t launch_bin_ks(const BinArgs& a, const double* k, const int* perm, double* ks, hipStream_t s);
+hipError_t launch_iota(int* p, long long n, hipStream_t s);
+
 // Node records (weights, local cell, caller index) of every work item.
 hipError_t launch_node_records(int D, bool dbl, int W, const KArgs& a, void* rec, hipStream_t s);
 

@@ -256,6 +256,25 @@ std::vector<double> poly_fit(int win, int m, double sigma) {
 // ---------------------------------------------------------------------------
 std::once_flag g_rocfft_once;
 
+// Work enqueued on `next` after a change of stream must not overtake work
+// still running on `cur` (both use the same plan-owned buffers): `next`
+// waits for everything enqueued on `cur` so far.  Skipped when either stream
+// is capturing a graph (a capture may not wait on outside events); callers
+// that capture order their streams themselves.
+int order_streams(hipStream_t cur, hipStream_t next, hipEvent_t* ev) {
+  if (next == cur) return NFFTM_OK;
+  hipStreamCaptureStatus a = hipStreamCaptureStatusNone, b = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(cur, &a) != hipSuccess || hipStreamIsCapturing(next, &b) != hipSuccess) {
+    (void)hipGetLastError();
+    return NFFTM_OK;
+  }
+  if (a != hipStreamCaptureStatusNone || b != hipStreamCaptureStatusNone) return NFFTM_OK;
+  if (!*ev) HIPCHK(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(*ev, cur));
+  HIPCHK(hipStreamWaitEvent(next, *ev, 0));
+  return NFFTM_OK;
+}
+
 struct FFTPass {
   rocfft_plan plan = nullptr;
   i64 nexec = 1, exec_stride = 0;  // executions at pointer offsets (elements)
@@ -457,6 +476,7 @@ struct nfftm_plan_s {
   };
   std::vector<GraphEntry> graphs;
   hipStream_t cap = nullptr;  // capture stream
+  hipEvent_t switch_ev = nullptr;  // orders work across nfftm_set_stream changes
   bool graphs_ok = true;      // false after a failed capture: direct launches only
   void drop_graphs() {
     for (auto& g : graphs)
@@ -467,6 +487,7 @@ struct nfftm_plan_s {
   ~nfftm_plan_s() {
     drop_graphs();
     if (cap) (void)hipStreamDestroy(cap);
+    if (switch_ev) (void)hipEventDestroy(switch_ev);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& pr : prof_ev) {
@@ -749,9 +770,12 @@ int check_nodes(const double* k, i64 n) {
   return NFFTM_OK;
 }
 
-// Node-dependent state (nodes!, src/implementation.jl:108-141): shift into
-// [0, 1) (src/utils.jl:32-44), bin by tile (src/precomputation.jl:487-520 at
-// tile granularity), split each tile's nodes into work items.
+// Node-dependent state (nodes!, src/implementation.jl:108-141): validation
+// on the host (checkNodes, src/utils.jl:46-55, before any device work), then
+// on the device (k_bin.hip): shift into [0, 1) (src/utils.jl:32-44), bin by
+// tile (src/precomputation.jl:459-520 at tile granularity) with a stable
+// sort, split each tile's nodes into work items, gather the shifted nodes in
+// tile order.
 int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
   const int D = p.D;
   if (M < 0 || M > 0x7fffffff) return fail(NFFTM_ERR_UNSUPPORTED, "M = %lld out of range", (long long)M);
@@ -760,68 +784,64 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
   p.gen++;
   p.drop_graphs();
   if (k != p.k.data()) p.k.assign(k, k + (size_t)D * M);
-  std::vector<double> ks((size_t)D * M);
-  std::vector<int> perm;
-  std::vector<int4> items;
-  std::vector<int> tstart;
-  auto shifted = [&](i64 j, int d) {
-    double x = k[d + (size_t)D * j];
-    if (x < 0.0) x += 1.0;
-    if (x == 1.0) x -= 2.220446049250313e-16;  // eps(Float64)
-    return x;
-  };
-  if (!p.tiled) {
-    for (i64 j = 0; j < M; ++j)
-      for (int d = 0; d < D; ++d) ks[(size_t)d * M + j] = shifted(j, d);
-    perm.resize(M);
-    for (i64 j = 0; j < M; ++j) perm[j] = (int)j;
-  } else {
-    const i64 ntile = (i64)p.ntiles[0] * p.ntiles[1] * p.ntiles[2];
-    std::vector<int> tile_of(M);
-    std::vector<i64> cnt(ntile + 1, 0);
-    for (i64 j = 0; j < M; ++j) {
-      i64 t = 0, str = 1;
-      for (int d = 0; d < D; ++d) {
-        const double kscale = shifted(j, d) * (double)p.Nt[d];
-        i64 c = (i64)kscale;
-        if (c >= p.Nt[d]) c -= p.Nt[d];
-        t += (c / p.T[d]) * str;
-        str *= p.ntiles[d];
-      }
-      tile_of[j] = (int)t;
-      cnt[t + 1]++;
-    }
-    for (i64 t = 0; t < ntile; ++t) cnt[t + 1] += cnt[t];
-    perm.resize(M);
-    {
-      std::vector<i64> pos(cnt.begin(), cnt.end() - 1);
-      for (i64 j = 0; j < M; ++j) perm[pos[tile_of[j]]++] = (int)j;
-    }
-    for (i64 s = 0; s < M; ++s)
-      for (int d = 0; d < D; ++d) ks[(size_t)d * M + s] = shifted(perm[s], d);
-    // work items: chunks of at most CH nodes of one tile
-    const i64 CH = D == 3 ? ITEM3 : D == 2 ? 2048 : 4096;
-    tstart.assign(ntile + 1, 0);
-    for (i64 t = 0; t < ntile; ++t) {
-      tstart[t] = (int)items.size();
-      for (i64 b0 = cnt[t]; b0 < cnt[t + 1]; b0 += CH) {
-        const i64 e0 = std::min(cnt[t + 1], b0 + CH);
-        items.push_back(int4{(int)t, (int)b0, (int)e0, 0});
-      }
-    }
-    tstart[ntile] = (int)items.size();
+  hipStream_t st = p.stream;
+  BinArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.D = D;
+  a.M = M;
+  i64 ntile = 1;
+  for (int d = 0; d < 3; ++d) {
+    a.Nt[d] = d < D ? (int)p.Nt[d] : 1;
+    a.T[d] = p.tiled && d < D ? p.T[d] : 1;
+    a.ntiles[d] = p.tiled && d < D ? p.ntiles[d] : 1;
+    ntile *= a.ntiles[d];
   }
-  p.nitems = (int)items.size();
-  p.single_items = p.tiled && p.nitems > 0;
-  for (size_t t = 0; p.single_items && t + 1 < tstart.size(); ++t) p.single_items = tstart[t + 1] - tstart[t] <= 1;
-  HIPCHK(p.d_ks.alloc(ks.size()));
-  HIPCHK(p.d_perm.alloc(perm.size()));
-  HIPCHK(p.d_items.alloc(items.size()));
-  if (!ks.empty()) HIPCHK(hipMemcpy(p.d_ks.p, ks.data(), ks.size() * sizeof(double), hipMemcpyHostToDevice));
-  if (!perm.empty())
-    HIPCHK(hipMemcpy(p.d_perm.p, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice));
-  if (!items.empty())
-    HIPCHK(hipMemcpy(p.d_items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+  a.ntile = (int)ntile;
+  a.CH = D == 3 ? ITEM3 : D == 2 ? 2048 : 4096;
+  a.key_bits = 1;
+  while (((i64)1 << a.key_bits) < ntile) ++a.key_bits;
+  DevBuf<double> kd;
+  HIPCHK(kd.alloc((size_t)D * M));
+  if (M) HIPCHK(hipMemcpyAsync(kd.p, k, (size_t)D * M * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(p.d_ks.alloc((size_t)D * M));
+  HIPCHK(p.d_perm.alloc((size_t)M));
+  int nitems = 0, most = 0;
+  if (!p.tiled) {
+    HIPCHK(launch_iota(p.d_perm.p, M, st));
+    HIPCHK(launch_bin_ks(a, kd.p, nullptr, p.d_ks.p, st));
+    HIPCHK(p.d_items.alloc(0));
+    HIPCHK(hipStreamSynchronize(st));  // before kd is freed
+  } else {
+    DevBuf<int> key_in, key_out, idx_in, cnt, off, ioff, stats;
+    HIPCHK(key_in.alloc((size_t)M));
+    HIPCHK(key_out.alloc((size_t)M));
+    HIPCHK(idx_in.alloc((size_t)M));
+    HIPCHK(cnt.alloc((size_t)ntile + 1));
+    HIPCHK(off.alloc((size_t)ntile + 1));
+    HIPCHK(ioff.alloc((size_t)ntile + 1));
+    HIPCHK(stats.alloc(2));
+    HIPCHK(launch_bin_keys(a, kd.p, key_in.p, idx_in.p, cnt.p, st));
+    DevBuf<char> tmp;
+    if (M) {
+      const size_t tb = bin_sort_temp_bytes(M, a.key_bits);
+      if (!tb) return fail(NFFTM_ERR_HIP, "radix sort of %lld nodes: temporary size query failed", (long long)M);
+      HIPCHK(tmp.alloc(tb));
+      HIPCHK(launch_bin_sort(a, tmp.p, tb, key_in.p, key_out.p, idx_in.p, p.d_perm.p, st));
+    }
+    HIPCHK(launch_bin_scan(a, cnt.p, off.p, ioff.p, stats.p, st));
+    int hs[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(hs, stats.p, sizeof hs, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    nitems = hs[0];
+    most = hs[1];
+    HIPCHK(p.d_items.alloc((size_t)nitems));
+    HIPCHK(p.d_tstart.alloc((size_t)ntile + 1));
+    HIPCHK(launch_bin_items(a, off.p, ioff.p, p.d_items.p, p.d_tstart.p, st));
+    HIPCHK(launch_bin_ks(a, kd.p, p.d_perm.p, p.d_ks.p, st));
+    HIPCHK(hipStreamSynchronize(st));  // before the temporaries are freed
+  }
+  p.nitems = nitems;
+  p.single_items = p.tiled && nitems > 0 && most <= 1;
   if (p.sparse && M > 0) {
     i64 K = 1;
     for (int d = 0; d < D; ++d) K *= p.W;
@@ -835,9 +855,7 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
     HIPCHK(launch_sparse_build(p.dbl(), p.W, p.kargs(), p.stream, p.d_bcols.p, p.d_bvals.p));
   }
   if (p.tiled) {
-    HIPCHK(p.d_tstart.alloc(tstart.size()));
-    HIPCHK(hipMemcpy(p.d_tstart.p, tstart.data(), tstart.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(p.d_scratch.alloc(std::max<size_t>(1, items.size()) * p.slot_cells() * p.celem()));
+    HIPCHK(p.d_scratch.alloc(std::max<size_t>(1, (size_t)nitems) * p.slot_cells() * p.celem()));
     // node records in tile order, built on the device
     const size_t rb = (size_t)node_rec_bytes(D, p.dbl() ? sizeof(double) : sizeof(float), p.W);
     HIPCHK(p.d_rec.alloc(std::max<size_t>(1, (size_t)M) * rb));
@@ -1200,27 +1218,37 @@ int tune_plan(nfftm_plan_s& p, int maxNT, std::vector<TuneChoice>* log) {
   TuneChoice best;
   best.ms = 1e30;
   int binned = -1;
+  // every exit, failed or not, leaves the plan binned on all of its nodes
+  auto restore = [&](int rc) {
+    if (rc != NFFTM_OK || best.NT == 0) {
+      choose_tiles(p);
+    } else {
+      if (p.D == 3) set_tile3(p, best.t);
+      else set_tile_shape(p, best.t);
+      p.NT = best.NT;
+    }
+    const std::string err = g_err;
+    const int rc2 = set_nodes(p, kfull.data(), Mfull);
+    if (rc != NFFTM_OK) {
+      g_err = err;  // report the tuning failure, not the restore
+      return rc;
+    }
+    return rc2;
+  };
   for (auto& c : cand) {
     const bool ok = p.D == 3 ? set_tile3(p, c.first) : set_tile_shape(p, c.first);
     if (!ok || !tiles_ok(p)) continue;
     if (c.first != binned) {
-      if (int rc = set_nodes(p, ktune, Mtune)) return rc;
+      if (int rc = set_nodes(p, ktune, Mtune)) return restore(rc);
       binned = c.first;
     }
     p.NT = c.second;
     double ms = 0;
-    if (int rc = time_gridding(p, fh.p, &ms)) return rc;
+    if (int rc = time_gridding(p, fh.p, &ms)) return restore(rc);
     if (log) log->push_back(TuneChoice{c.first, c.second, ms});
     if (ms < best.ms) best = TuneChoice{c.first, c.second, ms};
   }
-  if (best.NT == 0) {  // nothing fitted: restore the default shape
-    choose_tiles(p);
-    return set_nodes(p, kfull.data(), Mfull);
-  }
-  if (p.D == 3) set_tile3(p, best.t);
-  else set_tile_shape(p, best.t);
-  p.NT = best.NT;
-  return set_nodes(p, kfull.data(), Mfull);
+  return restore(NFFTM_OK);
 }
 
 }  // namespace
@@ -1244,6 +1272,9 @@ struct nfftm_batch_s {
   DevBuf<void*> d_grid_ptr, d_tab;  // d_tab: S input then S output pointers of a call
   void** h_tab = nullptr;           // pinned staging of d_tab
   hipEvent_t tab_done = nullptr;
+  hipStream_t last = nullptr;      // stream of the previous call
+  bool called = false;
+  hipEvent_t switch_ev = nullptr;  // orders calls on different streams
   struct TileGroup {
     int D = 0, W = 0, NT = 0, nblk = 0, nwork = 0;
     size_t lds = 0;
@@ -1271,6 +1302,7 @@ struct nfftm_batch_s {
       if (graph[i]) (void)hipGraphDestroy(graph[i]);
     }
     if (tab_done) (void)hipEventDestroy(tab_done);
+    if (switch_ev) (void)hipEventDestroy(switch_ev);
     if (h_tab) (void)hipHostFree(h_tab);
     if (cap) (void)hipStreamDestroy(cap);
   }
@@ -1493,6 +1525,10 @@ int batch_run(nfftm_batch_s& b, bool forward, const void* const* in, void* const
       return fail(NFFTM_ERR_ARG, "plan %d of the batch changed its nodes; create the batch again", (int)i);
   for (int i = 0; i < b.S; ++i)
     if (!in[i] || !out[i]) return fail(NFFTM_ERR_ARG, "NULL data pointer for slot %d", i);
+  if (b.called)
+    if (int rc = order_streams(b.last, s, &b.switch_ev)) return rc;
+  b.last = s;
+  b.called = true;
   HIPCHK(hipEventSynchronize(b.tab_done));  // the previous call has read the staging table
   for (int i = 0; i < b.S; ++i) {
     b.h_tab[i] = const_cast<void*>(in[i]);
@@ -1777,6 +1813,7 @@ int nfftm_profile_read(nfftm_plan p, double ms[8], int64_t count[8]) {
 int nfftm_set_stream(nfftm_plan p, void* s) {
   g_err.clear();
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if (int rc = order_streams(p->stream, (hipStream_t)s, &p->switch_ev)) return rc;
   p->stream = (hipStream_t)s;
   return NFFTM_OK;
 }
@@ -1806,6 +1843,18 @@ void nfftm_plan_destroy(nfftm_plan p) {
   if (!p) return;
   (void)hipStreamSynchronize(p->stream);
   delete p;
+}
+
+int nfftm_binning(nfftm_plan p, int* nitems, int32_t* perm_host, int32_t* items_host) {
+  g_err.clear();
+  if (!p || !nitems) return fail(NFFTM_ERR_ARG, "NULL argument");
+  *nitems = p->nitems;
+  HIPCHK(hipStreamSynchronize(p->stream));
+  if (perm_host && p->M)
+    HIPCHK(hipMemcpy(perm_host, p->d_perm.p, (size_t)p->M * sizeof(int), hipMemcpyDeviceToHost));
+  if (items_host && p->nitems)
+    HIPCHK(hipMemcpy(items_host, p->d_items.p, (size_t)p->nitems * sizeof(int4), hipMemcpyDeviceToHost));
+  return NFFTM_OK;
 }
 
 int nfftm_plan_tune(nfftm_plan p) {

@@ -447,6 +447,18 @@ class NFFTPlan:
         _check(self._lib.nfftm_plan_set_tiles(self._handle, int(tile), int(threads)))
         return self.tile_shape()
 
+    def binning(self):
+        """(perm, items) of the plan's node binning (nfftm_binning): perm[s] is
+        the caller index of the node at sorted position s; items is an
+        (nitems, 4) array of (tile, begin, end, 0) in Julia tile order."""
+        n = ctypes.c_int()
+        _check(self._lib.nfftm_binning(self._handle, ctypes.byref(n), None, None))
+        perm = np.empty(self.J, dtype=np.int32)
+        items = np.empty((n.value, 4), dtype=np.int32)
+        _check(self._lib.nfftm_binning(self._handle, ctypes.byref(n), perm.ctypes.data_as(ctypes.c_void_p),
+                                       items.ctypes.data_as(ctypes.c_void_p)))
+        return perm, items
+
     def tile_shape(self):
         """(tile edge per dimension, workgroup size) of the gridding kernels."""
         t = (ctypes.c_int * 3)()
