@@ -267,6 +267,19 @@ void nfftm_batch_destroy(nfftm_batch b);
  * Synchronises the plan's stream. */
 int nfftm_sdc(nfftm_plan p, int iters, void* weights_dev, int scale);
 
+/* nfftm_sdc with no device allocation: no counterpart in the reference.  For
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * errors as nfftm_sdc, but its work arrays (2 M complex values, plus P + M
+ * with scale = 1, and 16.6 KB of scalars) live in a fixed 64 MiB device
+ * workspace held by each GPU the library runs on until the process ends.
+ * Sizes beyond it fail with NFFTM_ERR_ARG before any device work (FP32:
+ * M <= ~4.1e6 without the scale).  Every call on one GPU shares that GPU's
+ * workspace: calls that overlap on one GPU (other host threads, other
+ * streams) race.  The plan's own grid and tables were allocated when it was
+ * created. */
+int nfftm_sdc_noalloc(nfftm_plan p, int iters, void* weights_dev, int scale);
+
 /* Toeplitz embedding of A'A for an image of size N (D <= 3): an object
  * owning the FFTs of size 2N and one work array of that size.
  * nfftm_toeplitz_kernel: lambda = FFT(fftshift(adjoint(p) * ones(M))) for a

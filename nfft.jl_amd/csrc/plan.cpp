@@ -1931,55 +1931,129 @@ void nfftm_batch_destroy(nfftm_batch b) {
 // ---------------------------------------------------------------------------
 // NFFTTools: sampling density compensation and the Toeplitz embedding.
 // ---------------------------------------------------------------------------
-int nfftm_sdc(nfftm_plan p, int iters, void* weights, int scale) {
-  g_err.clear();
-  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
-  if (iters < 0) return fail(NFFTM_ERR_ARG, "iters = %d < 0", iters);
-  if (!weights && p->M) return fail(NFFTM_ERR_ARG, "NULL weights");
-  if (p->M == 0) return NFFTM_OK;
+}  // extern "C"
+
+// The no-allocation workspace (nfftm_sdc_noalloc): a static device array,
+// one copy per GPU, whose address on each device is taken once with
+// hipGetSymbolAddress (that device current) and kept; host code hands only
+// that address to kernels and copies.
+constexpr size_t NOALLOC_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+namespace {
+
+// Work arrays of sdc!: the weights as complex values (wc) and their
+// interpolation (wt), M each; scalars sc[0] (first grid maximum), sc[1]
+// (sum Re v), sc[2] (sum |v|^2) then 2048 partials; the error flag; and for
+// the final scale an image (u, P) and a node vector (f, M).
+struct SdcWork {
+  void *wc, *wt, *u, *f;
+  double* sc;
+  int* bad;
+};
+
+int sdc_impl(nfftm_plan_s* p, int iters, void* weights, int scale, const SdcWork& w) {
   const bool dbl = p->dbl();
-  const size_t ce = p->celem();
   const i64 M = p->M;
   hipStream_t s = p->stream;
-  DevBuf<char> wc, wt;
-  DevBuf<double> sc;  // [0] first grid maximum, [1] sum Re v, [2] sum |v|^2, then 2048 partials
-  DevBuf<int> bad;
-  HIPCHK(wc.alloc((size_t)M * ce));
-  HIPCHK(wt.alloc((size_t)M * ce));
-  HIPCHK(sc.alloc(8 + 2048));
-  HIPCHK(bad.alloc(1));
-  HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-  double* part = sc.p + 8;
-  HIPCHK(launch_real_to_cpx(dbl, weights, wc.p, M, s));
+  double* part = w.sc + 8;
+  HIPCHK(launch_fill0(w.bad, sizeof(int), s));
+  HIPCHK(launch_real_to_cpx(dbl, weights, w.wc, M, s));
   // sdc!(p, iters, weights, weights_tmp, workg), samplingDensity.jl:93-118
   for (int i = 0; i < iters; ++i) {
-    if (int rc = run_conv(*p, true, wc.p, p->d_grid.p, true)) return rc;
-    if (i == 0) HIPCHK(launch_reduce(dbl, RED_MAX_RE, p->d_grid.p, p->Q, part, sc.p, s));
-    HIPCHK(launch_div_scalar(dbl, p->d_grid.p, p->Q, sc.p, s));
-    if (int rc = run_conv(*p, false, p->d_grid.p, wt.p, true)) return rc;
-    HIPCHK(launch_sdc_update(dbl, wc.p, wt.p, M, sc.p, bad.p, s));
+    if (int rc = run_conv(*p, true, w.wc, p->d_grid.p, true)) return rc;
+    if (i == 0) HIPCHK(launch_reduce(dbl, RED_MAX_RE, p->d_grid.p, p->Q, part, w.sc, s));
+    HIPCHK(launch_div_scalar(dbl, p->d_grid.p, p->Q, w.sc, s));
+    if (int rc = run_conv(*p, false, p->d_grid.p, w.wt, true)) return rc;
+    HIPCHK(launch_sdc_update(dbl, w.wc, w.wt, M, w.sc, w.bad, s));
   }
   if (scale) {
     // c = real(sum(v)) / sum(abs2, v), v = A' D(w) A 1 (samplingDensity.jl:141-154)
-    DevBuf<char> u, f;
-    HIPCHK(u.alloc((size_t)p->P * ce));
-    HIPCHK(f.alloc((size_t)M * ce));
-    HIPCHK(launch_fill_one(dbl, u.p, p->P, s));
-    if (int rc = enqueue_transform(*p, true, u.p, f.p)) return rc;
-    HIPCHK(launch_mul_real(dbl, f.p, wc.p, M, s));
-    if (int rc = enqueue_transform(*p, false, f.p, u.p)) return rc;
-    HIPCHK(launch_reduce(dbl, RED_SUM_RE, u.p, p->P, part, sc.p + 1, s));
-    HIPCHK(launch_reduce(dbl, RED_SUM_ABS2, u.p, p->P, part, sc.p + 2, s));
-    HIPCHK(launch_cpx_to_real(dbl, wc.p, weights, M, sc.p + 1, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(launch_fill_one(dbl, w.u, p->P, s));
+    if (int rc = enqueue_transform(*p, true, w.u, w.f)) return rc;
+    HIPCHK(launch_mul_real(dbl, w.f, w.wc, M, s));
+    if (int rc = enqueue_transform(*p, false, w.f, w.u)) return rc;
+    HIPCHK(launch_reduce(dbl, RED_SUM_RE, w.u, p->P, part, w.sc + 1, s));
+    HIPCHK(launch_reduce(dbl, RED_SUM_ABS2, w.u, p->P, part, w.sc + 2, s));
+    HIPCHK(launch_cpx_to_real(dbl, w.wc, weights, M, w.sc + 1, s));
   } else {
-    HIPCHK(launch_cpx_to_real(dbl, wc.p, weights, M, nullptr, s));
+    HIPCHK(launch_cpx_to_real(dbl, w.wc, weights, M, nullptr, s));
   }
   int hbad = 0;
-  HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&hbad, w.bad, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (hbad) return fail(NFFTM_ERR_UNSUPPORTED, "non-positive weights");
   return NFFTM_OK;
+}
+
+int sdc_check(nfftm_plan_s* p, int iters, void* weights) {
+  if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
+  if (iters < 0) return fail(NFFTM_ERR_ARG, "iters = %d < 0", iters);
+  if (!weights && p->M) return fail(NFFTM_ERR_ARG, "NULL weights");
+  return NFFTM_OK;
+}
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_address(void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(NFFTM_ERR_UNSUPPORTED, "device %d beyond %d", dev, MAX_DEVICES);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(NFFTM_ERR_HIP, "no device address for the workspace");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return NFFTM_OK;
+}
+
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+int nfftm_sdc(nfftm_plan p, int iters, void* weights, int scale) {
+  g_err.clear();
+  if (int rc = sdc_check(p, iters, weights)) return rc;
+  if (p->M == 0) return NFFTM_OK;
+  const size_t ce = p->celem();
+  DevBuf<char> wc, wt, u, f;
+  DevBuf<double> sc;
+  DevBuf<int> bad;
+  HIPCHK(wc.alloc((size_t)p->M * ce));
+  HIPCHK(wt.alloc((size_t)p->M * ce));
+  HIPCHK(sc.alloc(8 + 2048));
+  HIPCHK(bad.alloc(1));
+  if (scale) {
+    HIPCHK(u.alloc((size_t)p->P * ce));
+    HIPCHK(f.alloc((size_t)p->M * ce));
+  }
+  return sdc_impl(p, iters, weights, scale, SdcWork{wc.p, wt.p, u.p, f.p, sc.p, bad.p});
+}
+
+int nfftm_sdc_noalloc(nfftm_plan p, int iters, void* weights, int scale) {
+  g_err.clear();
+  if (int rc = sdc_check(p, iters, weights)) return rc;
+  if (p->M == 0) return NFFTM_OK;
+  const size_t ce = p->celem(), vM = align256((size_t)p->M * ce);
+  const size_t o_bad = align256((8 + 2048) * sizeof(double)), o_wc = o_bad + 256, o_wt = o_wc + vM,
+               o_u = o_wt + vM, o_f = o_u + (scale ? align256((size_t)p->P * ce) : 0),
+               need = o_f + (scale ? vM : 0);
+  if (need > NOALLOC_BYTES)
+    return fail(NFFTM_ERR_ARG, "sdc without allocation needs %zu bytes of workspace (M = %lld, %s), more than its %zu",
+                need, (long long)p->M, scale ? "with the scale" : "no scale", NOALLOC_BYTES);
+  void* ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  char* b = (char*)ws;
+  return sdc_impl(p, iters, weights, scale,
+                  SdcWork{b + o_wc, b + o_wt, scale ? b + o_u : nullptr, scale ? b + o_f : nullptr, (double*)b,
+                          (int*)(b + o_bad)});
 }
 
 }  // extern "C"

@@ -27,7 +27,7 @@ EXPORTS = (
     "nfftm_timing", "nfftm_set_timing", "nfftm_profile", "nfftm_profile_read",
     "nfftm_set_stream", "nfftm_plan_copy", "nfftm_binning", "nfftm_grad_nodes", "nfftm_plan_tune",
     "nfftm_tile_shape", "nfftm_plan_set_tiles", "nfftm_batch_create", "nfftm_batch_info", "nfftm_batch_forward",
-    "nfftm_batch_adjoint", "nfftm_batch_destroy", "nfftm_sdc", "nfftm_toeplitz_create",
+    "nfftm_batch_adjoint", "nfftm_batch_destroy", "nfftm_sdc", "nfftm_sdc_noalloc", "nfftm_toeplitz_create",
     "nfftm_toeplitz_kernel", "nfftm_toeplitz_apply", "nfftm_toeplitz_destroy",
     "nfftm_plan_destroy", "nfftm_last_error", "nfftm_version",
 )
@@ -65,6 +65,7 @@ def load():
     L.nfftm_batch_destroy.argtypes = [vp]
     L.nfftm_batch_destroy.restype = None
     L.nfftm_sdc.argtypes = [vp, i, vp, i]
+    L.nfftm_sdc_noalloc.argtypes = [vp, i, vp, i]
     L.nfftm_toeplitz_create.argtypes = [ctypes.POINTER(vp), i, i, pi64, vp]
     L.nfftm_toeplitz_kernel.argtypes = [vp, vp, vp]
     L.nfftm_toeplitz_apply.argtypes = [vp, vp, vp]
