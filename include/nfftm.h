@@ -109,6 +109,25 @@ int nfftm_plan_create(nfftm_plan* out, int prec, int D, const int64_t* N, int64_
                       const double* k_host, int m, double sigma, int window, int precompute,
                       int flags, void* hip_stream);
 
+/* Directional plan: NFFTPlan(k, N; dims = dim_lo+1:dim_hi+1, ...) --
+ * src/implementation.jl:73-106 with initParams' dims handling
+ * (src/precomputation.jl:3-56) and the stage operators of
+ * src/directional.jl:1-156 -- which the reference GPU extension refuses
+ * (ext/NFFTGPUArraysExt/implementation.jl:35-37).  N has D entries (Julia
+ * order), dims = dim_lo .. dim_hi (0-based) a range of them, and k_host is
+ * (dim_hi - dim_lo + 1) x M.  The transform runs over N[dims] for every
+ * slice along the other axes, in place in the caller's arrays (no
+ * permutation): f is prod(N) complex values, fHat is
+ * (N[0..dim_lo-1], M, N[dim_hi+1..D-1]) (NOut), and the stage operators take
+ * the grid (N[0..dim_lo-1], Ntilde[dims], N[dim_hi+1..]).  nfftm_sizes
+ * returns Ntilde over all D axes (N[d] off dims).  Typed stage operators,
+ * sdc, the node gradient, Toeplitz kernels and batches return
+ * NFFTM_ERR_UNSUPPORTED for such a plan.  dims covering all axes gives an
+ * ordinary plan. */
+int nfftm_plan_create_dims(nfftm_plan* out, int prec, int D, const int64_t* N, int dim_lo, int dim_hi,
+                           int64_t M, const double* k_host, int m, double sigma, int window, int precompute,
+                           int flags, void* hip_stream);
+
 /* nodes!(p, k): src/implementation.jl:108-141 (AbstractNFFTs/src/interface.jl:211).
  * M may differ from the plan's current M. */
 int nfftm_nodes(nfftm_plan p, const double* k_host, int64_t M);
@@ -304,6 +323,15 @@ int nfftm_plan_copy(nfftm_plan src, nfftm_plan* dst);
 
 /* finalizer (Wrappers/FINUFFT.jl:52-56 pattern). */
 void nfftm_plan_destroy(nfftm_plan p);
+
+/* Measurement utility, no counterpart in the reference: the issue cost of
+ * the LDS instructions that bound the gridding taps, on the current device
+ * (a short microbenchmark, about 10 ms).  cycles[0..3]: shader-clock cycles
+ * per CU of one wavefront instruction of ds_read_b64 (interpolation taps),
+ * ds_read_b128, ds_add_u64 (FP32 3-D spreading taps) and ds_add_f64 (FP64
+ * spreading taps), conflict-free; clock_hz and cus: the clock and the CU
+ * count they are counted against.  Allocates and synchronises the device. */
+int nfftm_lds_issue_cycles(double cycles[4], double* clock_hz, int* cus);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* nfftm_last_error(void);

@@ -63,14 +63,14 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_fwd(FArgs a, cons
   if (q < nlines) {
     const int iy = q % N1, iz = q / N1;
     const double fr = line_factor(a, iy, iz, true);
-    const C* src = f + (size_t)q * N0;
+    const C* src = f + (size_t)q * N0 * a.ist;
     double fc[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int ix = image_idx(j0 + k * G::TL, N0, L, Na0);
       fc[k] = 0;
       if (ix >= 0) {
-        x[k] = src[ix];
+        x[k] = src[(size_t)ix * a.ist];
         fc[k] = a.fac[0][ix];
       }
     }
@@ -120,13 +120,13 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft_x_adj(FArgs a, cons
   if (q < nlines) {
     const int iy = q % N1, iz = q / N1;
     const double fr = line_factor(a, iy, iz, false) * a.oscale;
-    C* dst = f + (size_t)q * N0;
+    C* dst = f + (size_t)q * N0 * a.ist;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int ix = image_idx(j0 + k * G::TL, N0, L, Na0);
       if (ix >= 0) {
         const double s = a.fac[0][ix] * fr;
-        dst[ix] = C{(R)((double)x[k].x * s), (R)((double)x[k].y * s)};
+        dst[(size_t)ix * a.ist] = C{(R)((double)x[k].x * s), (R)((double)x[k].y * s)};
       }
     }
   }

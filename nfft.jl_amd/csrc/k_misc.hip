@@ -49,13 +49,13 @@ __device__ __forceinline__ void deconv_pair(const DArgs& a, long long i, const t
     if (inside) {
       const int i0 = img_index(x, N0, Nt0, Na0), i1 = img_index(x + 1, N0, Nt0, Na0);
       if (i0 >= 0) {
-        const C v = f[fo + i0];
+        const C v = f[fo + i0 * a.nstr[0]];
         const double s = a.fac[0][i0] * facr;
         o0.x = (R)((double)v.x * s);
         o0.y = (R)((double)v.y * s);
       }
       if (i1 >= 0) {
-        const C v = f[fo + i1];
+        const C v = f[fo + i1 * a.nstr[0]];
         const double s = a.fac[0][i1] * facr;
         o1.x = (R)((double)v.x * s);
         o1.y = (R)((double)v.y * s);
@@ -83,7 +83,7 @@ __device__ __forceinline__ void deconv_t_row(const DArgs& a, int row, const type
                                              typename cpx<R>::T* __restrict__ f, int i0, int step) {
   using C = typename cpx<R>::T;
   const int N0 = a.N[0], Na0 = a.Na[0];
-  long long go = 0;
+  long long go = 0, fo = 0;
   double facr = 1.0;
   int r = row;
   for (int d = 1; d < a.D; ++d) {
@@ -91,6 +91,7 @@ __device__ __forceinline__ void deconv_t_row(const DArgs& a, int row, const type
     r /= a.N[d];
     const int gd = id < a.Na[d] ? id - a.Na[d] + a.Nt[d] : id - a.Na[d];
     go += (long long)gd * a.gstr[d];
+    fo += (long long)id * a.nstr[d];
     facr *= a.fac[d][id];
   }
   for (int i = i0; i < N0; i += step) {
@@ -100,7 +101,7 @@ __device__ __forceinline__ void deconv_t_row(const DArgs& a, int row, const type
     C out;
     out.x = (R)((double)v.x * s);
     out.y = (R)((double)v.y * s);
-    f[(long long)row * N0 + i] = out;
+    f[fo + (long long)i * a.nstr[0]] = out;
   }
 }
 
@@ -167,7 +168,8 @@ __global__ __launch_bounds__(256) void k_spread_generic(KArgs a, const typename 
     const int cw = node_weights<R, W>(a, d, a.ks[(size_t)d * a.M + j], w[d]);
     base[d] = (long long)cw - a.m + 1;
   }
-  const C f = C{fhat[j].x * (R)a.oscale, fhat[j].y * (R)a.oscale};
+  const C fj = fhat[(long long)j * a.fst];
+  const C f = C{fj.x * (R)a.oscale, fj.y * (R)a.oscale};
   R* g = (R*)grid;
   int l[MAXD];
   for (int d = 0; d < MAXD; ++d) l[d] = 0;
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(256) void k_interp_generic(KArgs a, const typename 
     }
     if (d == a.D) break;
   }
-  fhat[j] = C{sr * (R)a.oscale, si * (R)a.oscale};
+  fhat[(long long)j * a.fst] = C{sr * (R)a.oscale, si * (R)a.oscale};
 }
 
 template <typename R>
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(256) void k_sparse_interp(KArgs a, int K, const int
   }
   sr = wave_sum64(sr);
   si = wave_sum64(si);
-  if (lane == 0) fhat[j] = C{sr * (R)a.oscale, si * (R)a.oscale};
+  if (lane == 0) fhat[(long long)j * a.fst] = C{sr * (R)a.oscale, si * (R)a.oscale};
 }
 
 template <typename R>
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(256) void k_sparse_spread(KArgs a, int K, const int
   const int j = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (j >= a.M) return;
   const size_t o = (size_t)j * K;
-  const C f = fhat[j];
+  const C f = fhat[(long long)j * a.fst];
   const R fr = f.x * (R)a.oscale, fi = f.y * (R)a.oscale;
   for (int l = lane; l < K; l += 64) {
     const R v = vals[o + l];
@@ -449,6 +451,28 @@ hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s) {
   const long long tail = (long long)(bytes - (size_t)n16 * 16);
   if (tail > 0)
     hipLaunchKernelGGL(k_fill0_tail, dim3(1), dim3(64), 0, s, (unsigned char*)p + n16 * 16, tail);
+  return hipGetLastError();
+}
+
+// dst[i * ds] = src[i * ss], i < n: one slice of a directional plan's grid
+// between the caller's strided array and the plan's contiguous grid.
+template <typename C>
+__global__ __launch_bounds__(256) void k_copy_strided(const C* __restrict__ src, long long ss, C* __restrict__ dst,
+                                                      long long ds, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    dst[i * ds] = src[i * ss];
+}
+
+hipError_t launch_copy_strided(bool dbl, const void* src, long long ss, void* dst, long long ds, long long n,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+  if (dbl)
+    hipLaunchKernelGGL(k_copy_strided<double2>, dim3(blocks), dim3(256), 0, s, (const double2*)src, ss,
+                       (double2*)dst, ds, n);
+  else
+    hipLaunchKernelGGL(k_copy_strided<float2>, dim3(blocks), dim3(256), 0, s, (const float2*)src, ss,
+                       (float2*)dst, ds, n);
   return hipGetLastError();
 }
 

@@ -264,7 +264,7 @@ void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
       }
       ar = group_sum<16>(ar);
       ai = group_sum<16>(ai);
-      if (live && r16 == 0) fhat[rec_pj<R, 3, W>(lrec, n)] = C{ar * osc, ai * osc};
+      if (live && r16 == 0) fhat[(long long)rec_pj<R, 3, W>(lrec, n) * a.fst] = C{ar * osc, ai * osc};
     }
   }
 }
@@ -299,7 +299,7 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
   for (int h = 0; h < HPT; ++h) {
     const int j = it.y + (int)threadIdx.x + h * NT;
     C v{0, 0};
-    if (j < it.z) v = fhat[a.perm[j]];
+    if (j < it.z) v = fhat[(long long)a.perm[j] * a.fst];
     hre[h] = v.x;
     him[h] = v.y;
   }

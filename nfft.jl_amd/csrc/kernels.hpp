@@ -78,6 +78,8 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t x, const FastDiv& f) {
 struct KArgs {
   int D, m, W, mode;
   int M;                 // number of nodes
+  long long fst;         // element stride of the node values (directional plans: the product of the
+                         // sizes of the axes before the transformed ones; otherwise 1)
   int Nt[MAXD];          // oversampled grid size per dim
   double Ntd[MAXD];
   long long gstr[MAXD];  // grid strides (dim 1 fastest)
@@ -403,7 +405,7 @@ __device__ __forceinline__ void phase_a(const KArgs& a, const typename cpx<R>::T
   if (SPREAD) {
     for (int n = threadIdx.x; n < nb; n += NT) {
       const int jj = ((const int*)(a.rec + (size_t)(b0 + n) * N::BYTES + N::WBYTES))[1];
-      const typename cpx<R>::T v = fhat[jj];
+      const typename cpx<R>::T v = fhat[(long long)jj * a.fst];
       fv[n] = typename cpx<R>::T{v.x * scale, v.y * scale};
     }
   }
@@ -684,7 +686,8 @@ __device__ __forceinline__ void interp_tile_item(const KArgs& a, int item,
       }
       ar = group_sum<G>(ar);
       ai = group_sum<G>(ai);
-      if (lx == 0 && n < nb) fhat[rec_pj<R, D, W>(lrec, n)] = C{ar * (R)a.oscale, ai * (R)a.oscale};
+      if (lx == 0 && n < nb)
+        fhat[(long long)rec_pj<R, D, W>(lrec, n) * a.fst] = C{ar * (R)a.oscale, ai * (R)a.oscale};
     }
   }
 }

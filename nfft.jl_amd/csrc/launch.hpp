@@ -27,6 +27,7 @@ struct DArgs {
 // k < Nt).
 struct FArgs {
   int D;
+  long long ist;  // element stride of the image (directional plans; otherwise 1)
   int N[3], L[3], Na[3];
   long long gstr[3];
   const double* fac[3];
@@ -135,6 +136,9 @@ hipError_t launch_node_records(int D, bool dbl, int W, const KArgs& a, void* rec
 hipError_t launch_deconv(bool dbl, const DArgs& a, hipStream_t s, const void* f, void* g);
 hipError_t launch_deconv_t(bool dbl, const DArgs& a, hipStream_t s, const void* g, void* f);
 hipError_t launch_fill0(void* p, size_t bytes, hipStream_t s);
+// dst[i * ds] = src[i * ss] for n complex values of the precision
+hipError_t launch_copy_strided(bool dbl, const void* src, long long ss, void* dst, long long ds, long long n,
+                               hipStream_t s);
 
 // Sum the per-item padded tiles in a.scratch into the grid (D <= 3).
 hipError_t launch_fold(int D, bool dbl, int W, const KArgs& a, hipStream_t s, void* grid);
@@ -145,6 +149,11 @@ hipError_t launch_sparse_build(bool dbl, int W, const KArgs& a, hipStream_t s, i
 hipError_t launch_sparse(bool spread, bool dbl, const KArgs& a, int K, const int* cols, const void* vals,
                          hipStream_t s, const void* in, void* out);
 hipError_t launch_fold_d3(bool dbl, int W, const KArgs& a, hipStream_t s, void* grid);
+
+// Per-CU cycles of one wavefront instruction of ds_read_b64, ds_read_b128,
+// ds_add_u64 and ds_add_f64 on the current device (k_probe.hip), with the
+// shader clock and the CU count they are counted against.
+hipError_t lds_issue_cycles(double cycles[4], double* clock_hz, int* cus);
 
 // Compile-time tile geometry of the 3-D kernels: tile edge T, padded edge P,
 // dynamic LDS bytes.

@@ -404,6 +404,19 @@ struct nfftm_plan_s {
   double sigma_in = 2.0, sigma = 2.0, b = 0.0, wscale = 1.0;
   i64 N[MAXD] = {0}, Nt[MAXD] = {0}, P = 1, Q = 1, M = 0, LUTSize = 0;
   bool dbl() const { return prec == NFFTM_F64; }
+  // Directional plan (plan_nfft(k, N; dims), src/directional.jl): D, N, ...
+  // describe the transform over the axes dims = dlo .. dlo + D - 1 of the
+  // caller's Dfull-dimensional arrays (Julia order).  Every slice of those
+  // arrays along the other axes is one transform: its elements are Bsl apart
+  // (the product of the sizes before dims) and slices follow one another in
+  // Bsl x Asl order.  Dfull = 0: an ordinary plan.
+  int Dfull = 0, dlo = 0;
+  i64 Nfull[MAXD] = {0}, Bsl = 1, Asl = 1;
+  bool dir() const { return Dfull > 0; }
+  // element stride of the caller's image and node values in the kernels'
+  // arguments: Bsl while a directional transform runs, 1 otherwise (the
+  // plan's own work vectors are contiguous)
+  i64 io_stride = 1;
   // 3-D scratch slots in halo/interior block layout: whole tiles per
   // dimension and halos (m - 1, m cells) no wider than a tile
   bool blocked3() const {
@@ -575,6 +588,7 @@ struct nfftm_plan_s {
     a.scratch = d_scratch.p;
     a.tstart = d_tstart.p;
     a.ntile_all = ntiles[0] * ntiles[1] * ntiles[2];
+    a.fst = io_stride;
     return a;
   }
 
@@ -596,6 +610,7 @@ struct nfftm_plan_s {
       }
     }
     a.oscale = 1.0;
+    a.ist = io_stride;
     return a;
   }
 
@@ -608,7 +623,7 @@ struct nfftm_plan_s {
       a.N[d] = (int)N[d];
       a.Nt[d] = (int)Nt[d];
       a.Na[d] = (int)(N[d] / 2);
-      a.nstr[d] = ns;
+      a.nstr[d] = ns * io_stride;
       a.gstr[d] = gs;
       ns *= N[d];
       gs *= Nt[d];
@@ -982,6 +997,9 @@ int run_deconv(nfftm_plan_s& p, bool fwd, const void* in, void* out, bool standa
 // real part exactly.  Checked as the reference checks: complex input with
 // real output first (ArgumentError), then the sizes (DimensionMismatch).
 int typed_conv(nfftm_plan_s& p, bool spread, const void* in, int tin, i64 nin, void* out, int tout, i64 nout) {
+  if (p.dir())
+    return fail(NFFTM_ERR_UNSUPPORTED, "typed stage operators of a directional plan (use nfftm_convolve%s)",
+                spread ? "_transpose" : "");
   auto valid = [](int t) { return t >= NFFTM_R32 && t <= NFFTM_C16; };
   auto cpx = [](int t) { return t == NFFTM_C32 || t == NFFTM_C64 || t == NFFTM_C16; };
   if (!valid(tin) || !valid(tout)) return fail(NFFTM_ERR_ARG, "element type codes %d, %d not supported", tin, tout);
@@ -1079,16 +1097,17 @@ struct Timer {
     if (on) HIPCHK(hipEventRecord(p.ev[i], p.stream));
     return NFFTM_OK;
   }
-  int finish(double* t1, double* t2, double* t3) {
+  // add: accumulate (the slices of a directional transform) instead of set
+  int finish(double* t1, double* t2, double* t3, bool add = false) {
     if (!on) return NFFTM_OK;
     HIPCHK(hipEventSynchronize(p.ev[3]));
     float a = 0, b = 0, c = 0;
     HIPCHK(hipEventElapsedTime(&a, p.ev[0], p.ev[1]));
     HIPCHK(hipEventElapsedTime(&b, p.ev[1], p.ev[2]));
     HIPCHK(hipEventElapsedTime(&c, p.ev[2], p.ev[3]));
-    *t1 = a * 1e-3;
-    *t2 = b * 1e-3;
-    *t3 = c * 1e-3;
+    *t1 = (add ? *t1 : 0.0) + a * 1e-3;
+    *t2 = (add ? *t2 : 0.0) + b * 1e-3;
+    *t3 = (add ? *t3 : 0.0) + c * 1e-3;
     return NFFTM_OK;
   }
 };
@@ -1354,6 +1373,7 @@ int batch_build(nfftm_batch_s& b, const nfftm_plan* plans, const int* ntrans, in
     nfftm_plan_s& p = *plans[i];
     if (p.dbl() != b.dbl) return fail(NFFTM_ERR_ARG, "plans of a batch must share the precision");
     if (p.D > 2) return fail(NFFTM_ERR_UNSUPPORTED, "plan %d: batches take 1-D and 2-D plans", i);
+    if (p.dir()) return fail(NFFTM_ERR_UNSUPPORTED, "plan %d: a directional plan batches its own slices", i);
     if (!p.tiled) return fail(NFFTM_ERR_UNSUPPORTED, "plan %d: grid too small for the tiled kernels", i);
     const int K = ntrans ? ntrans[i] : 1;
     if (K < 1) return fail(NFFTM_ERR_ARG, "ntrans[%d] = %d must be positive", i, K);
@@ -1553,6 +1573,13 @@ extern "C" {
 const char* nfftm_last_error(void) { return g_err.c_str(); }
 const char* nfftm_version(void) { return "nfftm 0.1 (gfx950)"; }
 
+int nfftm_lds_issue_cycles(double cycles[4], double* clock_hz, int* cus) {
+  g_err.clear();
+  if (!cycles || !clock_hz || !cus) return fail(NFFTM_ERR_ARG, "NULL argument");
+  HIPCHK(lds_issue_cycles(cycles, clock_hz, cus));
+  return NFFTM_OK;
+}
+
 int nfftm_plan_create(nfftm_plan* out, int prec, int D, const int64_t* N, int64_t M, const double* k_host,
                       int m, double sigma, int window, int precompute, int flags, void* hip_stream) {
   g_err.clear();
@@ -1592,6 +1619,35 @@ int nfftm_plan_create(nfftm_plan* out, int prec, int D, const int64_t* N, int64_
   return NFFTM_OK;
 }
 
+int nfftm_plan_create_dims(nfftm_plan* out, int prec, int D, const int64_t* N, int dim_lo, int dim_hi, int64_t M,
+                           const double* k_host, int m, double sigma, int window, int precompute, int flags,
+                           void* hip_stream) {
+  g_err.clear();
+  if (!out) return fail(NFFTM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (D < 1 || D > MAXD) return fail(NFFTM_ERR_UNSUPPORTED, "D = %d not supported (1..%d)", D, MAXD);
+  if (!N) return fail(NFFTM_ERR_ARG, "N is NULL");
+  for (int d = 0; d < D; ++d)
+    if (N[d] < 1) return fail(NFFTM_ERR_ARG, "N[%d] = %lld must be positive", d, (long long)N[d]);
+  if (dim_lo < 0 || dim_hi < dim_lo || dim_hi >= D)
+    return fail(NFFTM_ERR_ARG, "dims = %d:%d is not a range of the axes 0..%d", dim_lo, dim_hi, D - 1);
+  nfftm_plan p = nullptr;
+  if (int rc = nfftm_plan_create(&p, prec, dim_hi - dim_lo + 1, N + dim_lo, M, k_host, m, sigma, window,
+                                 precompute, flags, hip_stream))
+    return rc;
+  if (dim_hi - dim_lo + 1 < D) {
+    p->Dfull = D;
+    p->dlo = dim_lo;
+    for (int d = 0; d < D; ++d) {
+      p->Nfull[d] = N[d];
+      if (d < dim_lo) p->Bsl *= N[d];
+      if (d > dim_hi) p->Asl *= N[d];
+    }
+  }
+  *out = p;
+  return NFFTM_OK;
+}
+
 int nfftm_nodes(nfftm_plan p, const double* k_host, int64_t M) {
   g_err.clear();
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
@@ -1605,18 +1661,44 @@ int nfftm_nodes(nfftm_plan p, const double* k_host, int64_t M) {
 
 namespace {
 
+// fn(in slice, out slice) for every slice of a directional plan's caller
+// arrays (in_len and out_len elements per slice), with the kernels' element
+// stride set to the slice stride meanwhile: slice (b, a) starts at element
+// b + a * Bsl * len (src/directional.jl: the axes before dims, then the
+// transformed ones, then those after).  Ordinary plans: fn(in, out) once.
+template <typename F>
+int for_slices(nfftm_plan_s& p, const void* in, i64 in_len, void* out, i64 out_len, F&& fn) {
+  if (!p.dir()) return fn(in, out);
+  struct Reset {
+    nfftm_plan_s& p;
+    ~Reset() { p.io_stride = 1; }
+  } reset{p};
+  p.io_stride = p.Bsl;
+  const size_t ce = p.celem();
+  for (i64 a = 0; a < p.Asl; ++a)
+    for (i64 b = 0; b < p.Bsl; ++b) {
+      const void* is = in ? (const char*)in + (size_t)(b + a * p.Bsl * in_len) * ce : nullptr;
+      void* os = out ? (char*)out + (size_t)(b + a * p.Bsl * out_len) * ce : nullptr;
+      if (int rc = fn(is, os)) return rc;
+    }
+  return NFFTM_OK;
+}
+
 // Launch sequence of one direction on p.stream: forward = deconvolve, FFT,
-// interpolate; adjoint = spread (+ fold), backward FFT, deconvolve_transpose.
+// interpolate; adjoint = spread (+ fold), backward FFT, deconvolve_transpose;
+// for every slice of a directional plan.
 int enqueue_transform(nfftm_plan_s& p, bool fwd, const void* in, void* out) {
   auto none = [] { return (int)NFFTM_OK; };
-  if (fwd) {
-    if (int rc = forward_to_grid(p, in, none)) return rc;
-    if (p.M) return run_conv(p, false, p.d_grid.p, out);
-    return NFFTM_OK;
-  }
-  const bool fused = fused_fold(p);
-  if (int rc = run_conv(p, true, in, p.d_grid.p, false, !fused)) return rc;
-  return grid_to_image(p, out, none, fused);
+  return for_slices(p, in, fwd ? p.P : p.M, out, fwd ? p.M : p.P, [&](const void* i, void* o) -> int {
+    if (fwd) {
+      if (int rc = forward_to_grid(p, i, none)) return rc;
+      if (p.M) return run_conv(p, false, p.d_grid.p, o);
+      return NFFTM_OK;
+    }
+    const bool fused = fused_fold(p);
+    if (int rc = run_conv(p, true, i, p.d_grid.p, false, !fused)) return rc;
+    return grid_to_image(p, o, none, fused);
+  });
 }
 
 // Graph replay of enqueue_transform for a repeated (input, output) pair.
@@ -1669,15 +1751,19 @@ int nfftm_forward(nfftm_plan p, const void* f, void* fhat) {
   if ((!f && p->P) || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
   if ((p->flags & NFFTM_FLAG_GRAPHS) && !p->profiling && !(p->flags & NFFTM_FLAG_TIMING))
     return graph_transform(*p, true, f, fhat);
-  Timer t(*p);
-  if (int rc = t.mark(0)) return rc;
-  if (int rc = forward_to_grid(*p, f, [&] { return t.mark(1); })) return rc;
-  if (int rc = t.mark(2)) return rc;
-  if (p->M) {
-    if (int rc = run_conv(*p, false, p->d_grid.p, fhat)) return rc;
-  }
-  if (int rc = t.mark(3)) return rc;
-  return t.finish(&p->timing[3], &p->timing[2], &p->timing[1]);
+  const bool acc = p->dir();
+  if (acc) p->timing[1] = p->timing[2] = p->timing[3] = 0;
+  return for_slices(*p, f, p->P, fhat, p->M, [&](const void* fs, void* fhs) -> int {
+    Timer t(*p);
+    if (int rc = t.mark(0)) return rc;
+    if (int rc = forward_to_grid(*p, fs, [&] { return t.mark(1); })) return rc;
+    if (int rc = t.mark(2)) return rc;
+    if (p->M) {
+      if (int rc = run_conv(*p, false, p->d_grid.p, fhs)) return rc;
+    }
+    if (int rc = t.mark(3)) return rc;
+    return t.finish(&p->timing[3], &p->timing[2], &p->timing[1], acc);
+  });
 }
 
 int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
@@ -1686,14 +1772,18 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
   if ((!f && p->P) || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
   if ((p->flags & NFFTM_FLAG_GRAPHS) && !p->profiling && !(p->flags & NFFTM_FLAG_TIMING))
     return graph_transform(*p, false, fhat, f);
-  Timer t(*p);
-  if (int rc = t.mark(0)) return rc;
-  const bool fused = fused_fold(*p);
-  if (int rc = run_conv(*p, true, fhat, p->d_grid.p, false, !fused)) return rc;
-  if (int rc = t.mark(1)) return rc;
-  if (int rc = grid_to_image(*p, f, [&] { return t.mark(2); }, fused)) return rc;
-  if (int rc = t.mark(3)) return rc;
-  return t.finish(&p->timing[4], &p->timing[5], &p->timing[6]);
+  const bool acc = p->dir();
+  if (acc) p->timing[4] = p->timing[5] = p->timing[6] = 0;
+  return for_slices(*p, fhat, p->M, f, p->P, [&](const void* fhs, void* fs) -> int {
+    Timer t(*p);
+    if (int rc = t.mark(0)) return rc;
+    const bool fused = fused_fold(*p);
+    if (int rc = run_conv(*p, true, fhs, p->d_grid.p, false, !fused)) return rc;
+    if (int rc = t.mark(1)) return rc;
+    if (int rc = grid_to_image(*p, fs, [&] { return t.mark(2); }, fused)) return rc;
+    if (int rc = t.mark(3)) return rc;
+    return t.finish(&p->timing[4], &p->timing[5], &p->timing[6], acc);
+  });
 }
 
 int nfftm_grad_nodes(nfftm_plan p, const void* f, const void* ybar, double* kbar) {
@@ -1704,6 +1794,7 @@ int nfftm_grad_nodes(nfftm_plan p, const void* f, const void* ybar, double* kbar
                 "node gradient of window %d needs precompute TENSOR or POLYNOMIAL (the analytic window "
                 "derivative is built for Kaiser-Bessel)", p->window);
   if ((!f && p->P) || (!ybar && p->M) || (!kbar && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
+  if (p->dir()) return fail(NFFTM_ERR_UNSUPPORTED, "node gradient of a directional plan");
   if (int rc = forward_to_grid(*p, f, [] { return (int)NFFTM_OK; })) return rc;
   if (p->M) {
     ProfScope ps(*p, 7);
@@ -1712,29 +1803,58 @@ int nfftm_grad_nodes(nfftm_plan p, const void* f, const void* ybar, double* kbar
   return NFFTM_OK;
 }
 
+// The stage operators of a directional plan (src/directional.jl:1-156) take
+// the whole arrays: f and g along all axes, the grid oversampled along dims
+// only.  Each slice runs through the plan's own grid, copied from or to the
+// caller's strided grid slice.
+static int grid_in(nfftm_plan_s& p, const void* gs) {
+  HIPCHK(launch_copy_strided(p.dbl(), gs, p.Bsl, p.d_grid.p, 1, p.Q, p.stream));
+  return NFFTM_OK;
+}
+static int grid_out(nfftm_plan_s& p, void* gs) {
+  HIPCHK(launch_copy_strided(p.dbl(), p.d_grid.p, 1, gs, p.Bsl, p.Q, p.stream));
+  return NFFTM_OK;
+}
+
 int nfftm_deconvolve(nfftm_plan p, const void* f, void* g) {
   g_err.clear();
   if (!p || !f || !g) return fail(NFFTM_ERR_ARG, "NULL argument");
-  return run_deconv(*p, true, f, g, true);
+  if (!p->dir()) return run_deconv(*p, true, f, g, true);
+  return for_slices(*p, f, p->P, g, p->Q, [&](const void* fs, void* gs) -> int {
+    if (int rc = run_deconv(*p, true, fs, p->d_grid.p, true)) return rc;
+    return grid_out(*p, gs);
+  });
 }
 
 int nfftm_deconvolve_transpose(nfftm_plan p, const void* g, void* f) {
   g_err.clear();
   if (!p || !f || !g) return fail(NFFTM_ERR_ARG, "NULL argument");
-  return run_deconv(*p, false, g, f, true);
+  if (!p->dir()) return run_deconv(*p, false, g, f, true);
+  return for_slices(*p, g, p->Q, f, p->P, [&](const void* gs, void* fs) -> int {
+    if (int rc = grid_in(*p, gs)) return rc;
+    return run_deconv(*p, false, p->d_grid.p, fs, true);
+  });
 }
 
 int nfftm_convolve(nfftm_plan p, const void* g, void* fhat) {
   g_err.clear();
   if (!p || !g || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL argument");
   if (!p->M) return NFFTM_OK;
-  return run_conv(*p, false, g, fhat, true);
+  if (!p->dir()) return run_conv(*p, false, g, fhat, true);
+  return for_slices(*p, g, p->Q, fhat, p->M, [&](const void* gs, void* fhs) -> int {
+    if (int rc = grid_in(*p, gs)) return rc;
+    return run_conv(*p, false, p->d_grid.p, fhs, true);
+  });
 }
 
 int nfftm_convolve_transpose(nfftm_plan p, const void* fhat, void* g) {
   g_err.clear();
   if (!p || !g || (!fhat && p->M)) return fail(NFFTM_ERR_ARG, "NULL argument");
-  return run_conv(*p, true, fhat, g, true);
+  if (!p->dir()) return run_conv(*p, true, fhat, g, true);
+  return for_slices(*p, fhat, p->M, g, p->Q, [&](const void* fhs, void* gs) -> int {
+    if (int rc = run_conv(*p, true, fhs, p->d_grid.p, true)) return rc;
+    return grid_out(*p, gs);
+  });
 }
 
 int nfftm_convolve_typed(nfftm_plan p, const void* g, int g_type, int64_t g_len, void* fhat, int fhat_type,
@@ -1755,14 +1875,25 @@ int nfftm_fft(nfftm_plan p, void* g, int direction) {
   g_err.clear();
   if (!p || !g) return fail(NFFTM_ERR_ARG, "NULL argument");
   if (direction != -1 && direction != 1) return fail(NFFTM_ERR_ARG, "direction must be -1 or +1");
-  return p->fft->exec(direction == -1, g, p->celem(), p->stream);
+  if (!p->dir()) return p->fft->exec(direction == -1, g, p->celem(), p->stream);
+  return for_slices(*p, g, p->Q, g, p->Q, [&](const void* gs, void*) -> int {
+    if (int rc = grid_in(*p, gs)) return rc;
+    if (int rc = p->fft->exec(direction == -1, p->d_grid.p, p->celem(), p->stream)) return rc;
+    return grid_out(*p, (void*)gs);
+  });
 }
 
 int nfftm_sizes(nfftm_plan p, int64_t* Ntilde, int64_t* M, double* sigma) {
   g_err.clear();
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
-  if (Ntilde)
+  if (Ntilde && p->dir()) {
+    // Ntilde over all axes: oversampled along dims only (initParams,
+    // src/precomputation.jl:25-27)
+    for (int d = 0; d < p->Dfull; ++d)
+      Ntilde[d] = d >= p->dlo && d < p->dlo + p->D ? p->Nt[d - p->dlo] : p->Nfull[d];
+  } else if (Ntilde) {
     for (int d = 0; d < p->D; ++d) Ntilde[d] = p->Nt[d];
+  }
   if (M) *M = p->M;
   if (sigma) *sigma = p->sigma;
   return NFFTM_OK;
@@ -1833,6 +1964,11 @@ int nfftm_plan_copy(nfftm_plan src, nfftm_plan* dst) {
   p->M = src->M;
   p->stream = src->stream;
   for (int d = 0; d < src->D; ++d) p->N[d] = src->N[d];
+  p->Dfull = src->Dfull;
+  p->dlo = src->dlo;
+  p->Bsl = src->Bsl;
+  p->Asl = src->Asl;
+  for (int d = 0; d < MAXD; ++d) p->Nfull[d] = src->Nfull[d];
   if (int rc = create_impl(*p, src->k.data())) return rc;
   std::memcpy(p->timing, src->timing, sizeof p->timing);
   *dst = p.release();
@@ -1990,6 +2126,7 @@ int sdc_check(nfftm_plan_s* p, int iters, void* weights) {
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
   if (iters < 0) return fail(NFFTM_ERR_ARG, "iters = %d < 0", iters);
   if (!weights && p->M) return fail(NFFTM_ERR_ARG, "NULL weights");
+  if (p->dir()) return fail(NFFTM_ERR_UNSUPPORTED, "sampling density of a directional plan");
   return NFFTM_OK;
 }
 
@@ -2109,6 +2246,7 @@ int nfftm_toeplitz_create(nfftm_toeplitz* out, int prec, int D, const int64_t* N
 int nfftm_toeplitz_kernel(nfftm_toeplitz t, nfftm_plan p, void* lambda) {
   g_err.clear();
   if (!t || !p || !lambda) return fail(NFFTM_ERR_ARG, "NULL argument");
+  if (p->dir()) return fail(NFFTM_ERR_UNSUPPORTED, "Toeplitz kernel of a directional plan");
   if (p->D != t->D || p->prec != t->prec) return fail(NFFTM_ERR_DIM, "plan and Toeplitz object differ in D or precision");
   for (int d = 0; d < t->D; ++d)
     if (p->N[d] != t->L[d])

@@ -57,15 +57,21 @@ struct MI355XBackend <: AbstractNFFTBackend end
 activate!() = AbstractNFFTs.set_active_backend!(MI355XNFFT)
 backend() = MI355XBackend()
 
-mutable struct MI355XNFFTPlan{T,D} <: AbstractNFFTs.AbstractNFFTPlan{T,D,1}
+# R = 1, or D - length(dims) + 1 for a directional plan (dims ≠ 1:D), whose
+# output puts J in place of the transformed axes (src/precomputation.jl:38-48).
+mutable struct MI355XNFFTPlan{T,D,R} <: AbstractNFFTs.AbstractNFFTPlan{T,D,R}
   handle::Ptr{Cvoid}
   N::NTuple{D,Int64}
-  NOut::NTuple{1,Int64}
+  NOut::NTuple{R,Int64}
   J::Int64
   k::Matrix{T}
-  Ñ::NTuple{D,Int64}
-  tmpVec::ROCArray{Complex{T},D}    # wraps the plan-owned grid (no copy)
+  Ñ::NTuple{D,Int64}                # oversampled along dims only
+  dims::UnitRange{Int64}
+  tmpVec::ROCArray{Complex{T}}      # wraps the plan-owned grid (no copy); one slice for a directional plan
 end
+
+nout(N::NTuple{D,Int}, dims::UnitRange{Int64}, J) where {D} =
+  (N[1:first(dims)-1]..., J, N[last(dims)+1:D]...)
 
 precision_of(::Type{Float32}) = NFFTM_F32
 precision_of(::Type{Float64}) = NFFTM_F64
@@ -95,7 +101,10 @@ function AbstractNFFTs.plan_nfft(::MI355XBackend, ::Type{<:ROCArray}, k::Matrix{
     blocking=true, window=:kaiser_bessel, precompute=AbstractNFFTs.POLYNOMIAL,
     timing::Union{Nothing,TimingStats}=nothing, graphs=false, rocfft=false, fused_fold=true,
     sparse_matrix=false, kargs...) where {T<:Union{Float32,Float64},D}
-  dims != 1:D && error("MI355X NFFT does not work along directions right now!")
+  dims_ = dims isa Integer ? (dims:dims) : dims
+  (1 <= first(dims_) <= last(dims_) <= D) || throw(ArgumentError("dims = $(dims) is not a range of 1:$(D)"))
+  size(k, 1) == length(dims_) ||
+    throw(ArgumentError("Nodes x have dimension $(size(k, 1)) != $(length(dims_))"))
   reltol, m, σ = AbstractNFFTs.accuracyParams(; kargs...)
   sortNodes && (k = sortslices(k, dims=2))
   kd = Matrix{Float64}(k)
@@ -103,13 +112,22 @@ function AbstractNFFTs.plan_nfft(::MI355XBackend, ::Type{<:ROCArray}, k::Matrix{
   flags = (blocking ? NFFTM_FLAG_BLOCKING : Cint(0)) | (timing === nothing ? Cint(0) : NFFTM_FLAG_TIMING) |
           (graphs ? NFFTM_FLAG_GRAPHS : Cint(0)) | (rocfft ? NFFTM_FLAG_ROCFFT : Cint(0)) |
           (fused_fold ? Cint(0) : NFFTM_FLAG_SEPARATE_FOLD) | (sparse_matrix ? NFFTM_FLAG_SPARSE : Cint(0))
-  t = @elapsed check(ccall((:nfftm_plan_create, libnfftm), Cint,
+  t = @elapsed if dims_ == 1:D
+    check(ccall((:nfftm_plan_create, libnfftm), Cint,
       (Ref{Ptr{Cvoid}}, Cint, Cint, Ptr{Int64}, Int64, Ptr{Float64}, Cint, Cdouble, Cint, Cint, Cint, Ptr{Cvoid}),
       h, precision_of(T), D, collect(Int64, N), size(k, 2), kd, m, σ, window_id(window),
       Cint(Int(precompute)), flags, AMDGPU.stream().stream))
+  else  # directional (src/directional.jl): slices transformed in place, strided
+    check(ccall((:nfftm_plan_create_dims, libnfftm), Cint,
+      (Ref{Ptr{Cvoid}}, Cint, Cint, Ptr{Int64}, Cint, Cint, Int64, Ptr{Float64}, Cint, Cdouble, Cint, Cint, Cint,
+       Ptr{Cvoid}),
+      h, precision_of(T), D, collect(Int64, N), first(dims_) - 1, last(dims_) - 1, size(k, 2), kd, m, σ,
+      window_id(window), Cint(Int(precompute)), flags, AMDGPU.stream().stream))
+  end
   timing === nothing || (timing.pre = t)
   Ñ, M = sizes(h[], D)
-  p = MI355XNFFTPlan{T,D}(h[], N, (Int64(M),), Int64(M), k, Ñ, wrap_grid(h[], T, Ñ))
+  NOut = nout(N, dims_, Int64(M))
+  p = MI355XNFFTPlan{T,D,length(NOut)}(h[], N, NOut, Int64(M), k, Ñ, dims_, wrap_grid(h[], T, Ñ[dims_]))
   finalizer(p) do q   # Wrappers/FINUFFT.jl:52-56
     ccall((:nfftm_plan_destroy, libnfftm), Cvoid, (Ptr{Cvoid},), q.handle)
   end
@@ -123,14 +141,14 @@ AbstractNFFTs.size_out(p::MI355XNFFTPlan) = p.NOut
 function AbstractNFFTs.nodes!(p::MI355XNFFTPlan{T,D}, k::Matrix{T}) where {T,D}
   check(ccall((:nfftm_nodes, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64),
               p.handle, Matrix{Float64}(k), size(k, 2)))
-  p.k = k; p.J = size(k, 2); p.NOut = (p.J,)
+  p.k = k; p.J = size(k, 2); p.NOut = nout(p.N, p.dims, p.J)
   return p
 end
 
-function Base.copy(p::MI355XNFFTPlan{T,D}) where {T,D}   # src/implementation.jl:45-66
+function Base.copy(p::MI355XNFFTPlan{T,D,R}) where {T,D,R}   # src/implementation.jl:45-66
   h = Ref{Ptr{Cvoid}}(C_NULL)
   check(ccall((:nfftm_plan_copy, libnfftm), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}), p.handle, h))
-  q = MI355XNFFTPlan{T,D}(h[], p.N, p.NOut, p.J, copy(p.k), p.Ñ, wrap_grid(h[], T, p.Ñ))
+  q = MI355XNFFTPlan{T,D,R}(h[], p.N, p.NOut, p.J, copy(p.k), p.Ñ, p.dims, wrap_grid(h[], T, p.Ñ[p.dims]))
   finalizer(x -> ccall((:nfftm_plan_destroy, libnfftm), Cvoid, (Ptr{Cvoid},), x.handle), q)
   return q
 end
@@ -140,7 +158,7 @@ devptr(a::ROCArray) = Ptr{Cvoid}(pointer(a))
 # src/utils.jl:98-105
 function consistency(p::MI355XNFFTPlan{T,D}, f::ROCArray, fHat::ROCArray) where {T,D}
   size(f) == p.N || throw(DimensionMismatch("Data f has size $(size(f)) but the plan expects $(p.N)"))
-  size(fHat, 1) == p.J || throw(DimensionMismatch("fHat has length $(size(fHat,1)) but the plan has $(p.J) nodes"))
+  size(fHat) == p.NOut || throw(DimensionMismatch("fHat has size $(size(fHat)) but the plan expects $(p.NOut)"))
   (eltype(f) == Complex{T} && eltype(fHat) == Complex{T}) ||
     throw(ArgumentError("data must be Complex{$T} for this plan"))
 end
@@ -191,11 +209,18 @@ elt(::Type{E}) where {E} = haskey(ELT, E) ? ELT[E] :
 # complex of any precision (RealOrComplex, as NFFTTools.sdc! passes real
 # weights and grids, samplingDensity.jl:106,112); the library converts
 # through the plan's complex type and checks the lengths again.
-function AbstractNFFTs.convolve!(p::MI355XNFFTPlan{T,D}, g::ROCArray{<:Any,D}, fHat::ROCVector) where {T,D}
+function AbstractNFFTs.convolve!(p::MI355XNFFTPlan{T,D}, g::ROCArray{<:Any,D}, fHat::ROCArray) where {T,D}
   (eltype(g) <: Complex && !(eltype(fHat) <: Complex)) &&
     throw(ArgumentError("Complex input g requires Complex output fHat"))
   size(g) == p.Ñ || throw(DimensionMismatch("size(g)=$(size(g)) ≠ Ñ = $(p.Ñ)"))
-  size(fHat) == (p.J,) || throw(DimensionMismatch("size(fHat)=$(size(fHat)) ≠ J = $(p.J)"))
+  size(fHat) == p.NOut || throw(DimensionMismatch("size(fHat)=$(size(fHat)) ≠ $(p.NOut)"))
+  if p.dims != 1:D  # directional (src/directional.jl:57-101): the plan's complex type
+    (eltype(g) == Complex{T} && eltype(fHat) == Complex{T}) ||
+      throw(ArgumentError("a directional plan convolves Complex{$T} arrays"))
+    check(ccall((:nfftm_convolve, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                p.handle, devptr(g), devptr(fHat)))
+    return fHat
+  end
   check(ccall((:nfftm_convolve_typed, libnfftm), Cint,
               (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}, Cint, Int64),
               p.handle, devptr(g), elt(eltype(g)), length(g), devptr(fHat), elt(eltype(fHat)), length(fHat)))
@@ -203,11 +228,18 @@ function AbstractNFFTs.convolve!(p::MI355XNFFTPlan{T,D}, g::ROCArray{<:Any,D}, f
 end
 
 # convolve_transpose!(p, fHat, g): src/convolution.jl:115-149.
-function AbstractNFFTs.convolve_transpose!(p::MI355XNFFTPlan{T,D}, fHat::ROCVector, g::ROCArray{<:Any,D}) where {T,D}
+function AbstractNFFTs.convolve_transpose!(p::MI355XNFFTPlan{T,D}, fHat::ROCArray, g::ROCArray{<:Any,D}) where {T,D}
   (eltype(fHat) <: Complex && !(eltype(g) <: Complex)) &&
     throw(ArgumentError("Complex input fHat requires Complex output g"))
   size(g) == p.Ñ || throw(DimensionMismatch("size(g)=$(size(g)) ≠ Ñ = $(p.Ñ)"))
-  size(fHat) == (p.J,) || throw(DimensionMismatch("size(fHat)=$(size(fHat)) ≠ J = $(p.J)"))
+  size(fHat) == p.NOut || throw(DimensionMismatch("size(fHat)=$(size(fHat)) ≠ $(p.NOut)"))
+  if p.dims != 1:D  # directional (src/directional.jl:103-156)
+    (eltype(g) == Complex{T} && eltype(fHat) == Complex{T}) ||
+      throw(ArgumentError("a directional plan convolves Complex{$T} arrays"))
+    check(ccall((:nfftm_convolve_transpose, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                p.handle, devptr(fHat), devptr(g)))
+    return g
+  end
   check(ccall((:nfftm_convolve_transpose_typed, libnfftm), Cint,
               (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}, Cint, Int64),
               p.handle, devptr(fHat), elt(eltype(fHat)), length(fHat), devptr(g), elt(eltype(g)), length(g)))

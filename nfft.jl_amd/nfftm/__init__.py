@@ -234,10 +234,9 @@ class NFFTPlan:
         k = _as_nodes(k)
         N = (int(N),) if np.isscalar(N) else tuple(int(n) for n in N)
         D = len(N)
-        if dims is not None and tuple(np.atleast_1d(dims)) != tuple(range(D)):
-            raise NFFTError("GPU NFFT does not work along directions right now!")
-        if k.shape[0] != D:
-            raise ArgumentError(f"Nodes x have dimension {k.shape[0]} != {D}")
+        dims = _dims_range(dims, D)
+        if k.shape[0] != len(dims):
+            raise ArgumentError(f"Nodes x have dimension {k.shape[0]} != {len(dims)}")
         m, sigma, reltol = accuracy_params(**kw)
         window = str(kw.get("window", "kaiser_bessel")).lstrip(":")  # a Julia symbol name works too
         if window not in _WINDOWS:
@@ -259,13 +258,20 @@ class NFFTPlan:
                  | (_L.FLAG_ROCFFT if kw.get("rocfft", False) else 0)
                  | (0 if kw.get("fused_fold", True) else _L.FLAG_SEPARATE_FOLD)
                  | (_L.FLAG_SPARSE if kw.get("sparse_matrix", False) else 0))
-        _check(self._lib.nfftm_plan_create(
-            ctypes.byref(self._handle), prec, D, Nj, k.shape[1],
-            kj.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m, sigma, _WINDOWS[window],
-            precompute, flags, _stream()))
+        kp = kj.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        if len(dims) == D:
+            _check(self._lib.nfftm_plan_create(
+                ctypes.byref(self._handle), prec, D, Nj, k.shape[1], kp, m, sigma, _WINDOWS[window],
+                precompute, flags, _stream()))
+        else:
+            # Julia axes of the C-order axes dims: D - 1 - a
+            _check(self._lib.nfftm_plan_create_dims(
+                ctypes.byref(self._handle), prec, D, Nj, D - 1 - dims[-1], D - 1 - dims[0], k.shape[1], kp,
+                m, sigma, _WINDOWS[window], precompute, flags, _stream()))
         self.k = k
         self.N = N
         self.D = D
+        self.dims = dims
         self.prec = prec
         self.dtype = _ctype(prec)
         self.params = dict(m=m, sigma=sigma, reltol=reltol, window=window, precompute=precompute,
@@ -273,6 +279,15 @@ class NFFTPlan:
         self._read_sizes()
         if timing is not None:
             timing.pre = time.perf_counter() - t0
+
+    @property
+    def directional(self):
+        return len(self.dims) < self.D
+
+    @property
+    def Nsub(self):
+        """N along dims (the size of one transformed slice)."""
+        return tuple(self.N[a] for a in self.dims)
 
     def _read_sizes(self):
         Nt = (ctypes.c_int64 * self.D)()
@@ -290,7 +305,9 @@ class NFFTPlan:
 
     @property
     def NOut(self):
-        return (self.J,)
+        """Output size: J in place of the transformed axes (initParams,
+        src/precomputation.jl:38-48)."""
+        return self.N[:self.dims[0]] + (self.J,) + self.N[self.dims[-1] + 1:]
 
     @property
     def Ñ(self):
@@ -300,7 +317,7 @@ class NFFTPlan:
         return self.N
 
     def size_out(self):
-        return (self.J,)
+        return self.NOut
 
     @property
     def H(self):
@@ -319,8 +336,9 @@ class NFFTPlan:
             self._handle = None
 
     def __repr__(self):
-        return (f"NFFTPlan (MI355X) with {self.J} sampling points for an input array of size "
-                f"{self.N} and an output array of size {self.NOut}")
+        r = (f"NFFTPlan (MI355X) with {self.J} sampling points for an input array of size "
+             f"{self.N} and an output array of size {self.NOut}")
+        return r + (f" with dims {self.dims}" if self.directional else "")
 
     def copy(self):
         """copy(p), src/implementation.jl:45-66."""
@@ -328,7 +346,7 @@ class NFFTPlan:
         c._lib = self._lib
         c._handle = ctypes.c_void_p()
         _check(self._lib.nfftm_plan_copy(self._handle, ctypes.byref(c._handle)))
-        c.k, c.N, c.D, c.prec, c.dtype = self.k, self.N, self.D, self.prec, self.dtype
+        c.k, c.N, c.D, c.dims, c.prec, c.dtype = self.k, self.N, self.D, self.dims, self.prec, self.dtype
         c.params = dict(self.params)
         c._read_sizes()
         return c
@@ -336,8 +354,8 @@ class NFFTPlan:
     def nodes_(self, k):
         """nodes!(p, k), src/implementation.jl:108-141."""
         k = _as_nodes(k)
-        if k.shape[0] != self.D:
-            raise ArgumentError(f"Nodes x have dimension {k.shape[0]} != {self.D}")
+        if k.shape[0] != len(self.dims):
+            raise ArgumentError(f"Nodes x have dimension {k.shape[0]} != {len(self.dims)}")
         if self.params["sortNodes"]:
             order = np.lexsort(k)
             k[:] = k[:, order]
@@ -356,7 +374,8 @@ class NFFTPlan:
         ptr = ctypes.c_void_p()
         _check(self._lib.nfftm_grid(self._handle, ctypes.byref(ptr)))
         ts = "<c16" if self.prec == _L.F64 else "<c8"
-        return torch.as_tensor(_CudaArray(ptr.value, self.Ntilde, ts, self), device=_device())
+        shape = tuple(self.Ntilde[a] for a in self.dims)  # one slice of a directional plan
+        return torch.as_tensor(_CudaArray(ptr.value, shape, ts, self), device=_device())
 
     # transforms -----------------------------------------------------------------
     def _prep(self, x, shape, what):
@@ -400,7 +419,7 @@ class NFFTPlan:
     def forward(self, f, out=None, *, timing=None, verbose=False):
         """mul!(fHat, p, f), src/implementation.jl:155-172."""
         f = self._prep(f, self.N, "f")
-        o, user = self._out(out, (self.J,), "fHat")
+        o, user = self._out(out, self.NOut, "fHat")
         self._run(self._lib.nfftm_forward, f, o, timing, verbose, ("deconv", "fft", "conv"))
         if user is not None:
             user.copy_(o)
@@ -409,7 +428,7 @@ class NFFTPlan:
 
     def adjoint(self, fhat, out=None, *, timing=None, verbose=False):
         """mul!(f, adjoint(p), fHat), src/implementation.jl:176-193."""
-        fhat = self._prep(fhat, (self.J,), "fHat")
+        fhat = self._prep(fhat, self.NOut, "fHat")
         o, user = self._out(out, self.N, "f")
         self._run(self._lib.nfftm_adjoint, fhat, o, timing, verbose,
                   ("conv_adjoint", "fft_adjoint", "deconv_adjoint"))
@@ -423,6 +442,8 @@ class NFFTPlan:
         fHat = p * f: the node cotangent that the reference's rrule leaves as
         NoTangent() (AbstractNFFTs/ext/AbstractNFFTsChainRulesCoreExt.jl:14-45).
         Returns a (D, M) float64 device tensor in the axis order of k."""
+        if self.directional:  # as the library answers (NFFTM_ERR_UNSUPPORTED)
+            raise NFFTError("node gradient of a directional plan")
         f = self._prep(f, self.N, "f")
         ybar = self._prep(ybar, (self.J,), "ybar")
         kb = torch.empty((self.J, self.D), dtype=torch.float64, device=_device())
@@ -517,11 +538,16 @@ class NFFTPlan:
     def convolve_(self, g, fhat):
         """convolve!(p, g, fHat), src/convolution.jl:20-53 (g and fHat real or
         complex, any precision)."""
+        if self.directional:
+            return self._stage(self._lib.nfftm_convolve, g, self.Ntilde, fhat, self.NOut, ("g", "fHat"))
         return self._typed(self._lib.nfftm_convolve_typed, g, self.Ntilde, fhat, (self.J,), ("g", "fHat"))
 
     def convolve_transpose_(self, fhat, g):
         """convolve_transpose!(p, fHat, g), src/convolution.jl:115-149 (fHat and
         g real or complex, any precision)."""
+        if self.directional:
+            return self._stage(self._lib.nfftm_convolve_transpose, fhat, self.NOut, g, self.Ntilde,
+                               ("fHat", "g"))
         return self._typed(self._lib.nfftm_convolve_transpose_typed, fhat, (self.J,), g, self.Ntilde,
                            ("fHat", "g"))
 
@@ -666,122 +692,45 @@ def plan_nfft(*args, **kw):
     k, N = args[0], args[1]
     dims = kw.pop("dims", None)
     Nt = (int(N),) if np.isscalar(N) else tuple(int(n) for n in N)
-    if dims is not None and tuple(np.atleast_1d(dims)) != tuple(range(len(Nt))):
+    if dims is not None and _dims_range(dims, len(Nt)) != tuple(range(len(Nt))):
         return DirectionalPlan(k, Nt, dims, **kw)
     return NFFTPlan(k, N, **kw)
 
 
-class DirectionalPlan:
+def _dims_range(dims, D):
+    """dims (an int, a range or a sequence of consecutive C-order axes; None
+    for all) as a tuple of axes, checked as initParams checks them."""
+    if dims is None:
+        return tuple(range(D))
+    d = np.atleast_1d(np.asarray(list(dims) if isinstance(dims, range) else dims, dtype=np.int64))
+    if d.ndim != 1 or len(d) == 0 or np.any(np.diff(d) != 1) or d[0] < 0 or d[-1] >= D:
+        raise ArgumentError(f"dims must be a range of consecutive axes of a {D}-dimensional array")
+    return tuple(int(x) for x in d)
+
+
+class DirectionalPlan(NFFTPlan):
     """NFFT along a contiguous range of axes of a larger array
     (``plan_nfft(k, N; dims)``, src/directional.jl:1-156 with initParams'
-    dims handling at src/precomputation.jl:3-29).  The other axes are batch
+    dims handling at src/precomputation.jl:3-56).  The other axes are batch
     axes: the transform equals the ``len(dims)``-dimensional NFFT of every
     slice, and the output replaces the transformed axes by one axis of
     length J.  ``dims`` are axes of the C-order array (an int or a range of
     consecutive ints); node row d pairs with axis dims[d].
 
-    The slices are transformed by one plan over N[dims]: as one batch of
-    slices sharing its nodes (nfftm_batch_*) when len(dims) <= 2, slice by
-    slice otherwise.  Moving the batch axes aside is a device permutation."""
+    One C-ABI plan (nfftm_plan_create_dims) transforms every slice in place
+    in the caller's arrays: the kernels read and write the slices with their
+    strides, so no axis permutation is made.  Node gradients, sdc, Toeplitz
+    kernels, batches and the typed stage operators are not defined for it
+    (NFFTM_ERR_UNSUPPORTED)."""
 
     def __init__(self, k, N, dims, **kw):
-        d = np.atleast_1d(np.asarray(list(dims) if isinstance(dims, range) else dims, dtype=np.int64))
-        D = len(N)
-        if d.ndim != 1 or len(d) == 0 or np.any(np.diff(d) != 1) or d[0] < 0 or d[-1] >= D:
-            raise ArgumentError(f"dims must be a range of consecutive axes of a {D}-dimensional array")
-        k = _as_nodes(k)
-        if k.shape[0] != len(d):
-            raise ArgumentError(f"Nodes x have dimension {k.shape[0]} != {len(d)}")
-        self.N = tuple(N)
-        self.dims = tuple(int(x) for x in d)
-        self.D = D
-        self.sub = NFFTPlan(k, tuple(N[x] for x in self.dims), **kw)
-        self.J = self.sub.J
-        self.dtype = self.sub.dtype
-        self._pre = self.N[:self.dims[0]]
-        self._post = self.N[self.dims[-1] + 1:]
-        self.B = int(np.prod(self._pre + self._post, dtype=np.int64))
-        self.NOut = self._pre + (self.J,) + self._post
-        self._batch = self._make_batch()
+        N = (int(N),) if np.isscalar(N) else tuple(int(n) for n in N)
+        super().__init__(k, N, dims=dims, **kw)
 
-    def _make_batch(self):
-        # plans the batch kernels do not take (D = 3, node-parallel tiny
-        # grids) run slice by slice
-        if self.sub.D > 2 or self.B < 2:
-            return None
-        try:
-            return BatchPlan([self.sub], [self.B], autotune=False)
-        except NFFTError:
-            return None
-
-    # the batch axes first, then the transformed axes (image) or J (nodes)
-    def _to_slices(self, x, inner):
-        n0, n1 = len(self._pre), len(self._pre) + len(inner)
-        perm = list(range(n0)) + list(range(n1, x.dim())) + list(range(n0, n1))
-        return x.permute(perm).reshape((self.B,) + tuple(inner)).contiguous()
-
-    def _from_slices(self, y, inner):
-        y = y.reshape(self._pre + self._post + tuple(inner))
-        n0, nb = len(self._pre), len(self._pre) + len(self._post)
-        perm = list(range(n0)) + list(range(nb, nb + len(inner))) + list(range(n0, nb))
-        return y.permute(perm).contiguous()
-
-    def _run(self, x, shape_in, inner_in, inner_out, fwd):
-        x = _to_dev(x, self.dtype)
-        if tuple(x.shape) != tuple(shape_in):
-            raise DimensionMismatch(f"Data is not consistent with the directional plan (shape "
-                                    f"{tuple(x.shape)}, expected {tuple(shape_in)})")
-        xs = self._to_slices(x, inner_in)
-        ys = torch.empty((self.B,) + tuple(inner_out), dtype=self.dtype, device=_device())
-        if self._batch is not None:
-            fn = self._batch.forward if fwd else self._batch.adjoint
-            fn([xs[b] for b in range(self.B)], [ys[b] for b in range(self.B)])
-        else:
-            for b in range(self.B):
-                if fwd:
-                    self.sub.forward(xs[b], ys[b])
-                else:
-                    self.sub.adjoint(xs[b], ys[b])
-        return self._from_slices(ys, inner_out)
-
-    def forward(self, f, out=None):
-        """mul!(fHat, p, f) for a directional plan."""
-        y = self._run(f, self.N, self.sub.N, (self.J,), True)
-        if out is not None:
-            out.copy_(y)
-            return out
-        return y
-
-    def adjoint(self, fhat, out=None):
-        """mul!(f, adjoint(p), fHat) for a directional plan."""
-        x = self._run(fhat, self.NOut, (self.J,), self.sub.N, False)
-        if out is not None:
-            out.copy_(x)
-            return out
-        return x
-
-    def size_in(self):
-        return self.N
-
-    def size_out(self):
-        return self.NOut
-
-    @property
-    def H(self):
-        return AdjointPlan(self)
-
-    def __mul__(self, f):
-        return self.forward(f)
-
-    def nodes_(self, k):
-        """nodes!(p, k): new nodes for every slice (same J)."""
-        self.sub.nodes_(k)
-        self._batch = self._make_batch()
-        return self
-
-    def __repr__(self):
-        return (f"NFFTPlan with {self.J} sampling points for an input array of size{self.N} and an "
-                f"output array of size{self.NOut} with dims {self.dims}")
+    def copy(self):
+        c = super().copy()
+        c.__class__ = DirectionalPlan
+        return c
 
 
 def mul_(out, p, inp, **kw):

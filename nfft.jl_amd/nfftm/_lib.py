@@ -21,7 +21,7 @@ R32, R64, C32, C64, R16, C16 = range(6)  # element types of the typed stage oper
 
 # every symbol include/nfftm.h declares
 EXPORTS = (
-    "nfftm_plan_create", "nfftm_nodes", "nfftm_forward", "nfftm_adjoint",
+    "nfftm_plan_create", "nfftm_plan_create_dims", "nfftm_nodes", "nfftm_forward", "nfftm_adjoint",
     "nfftm_deconvolve", "nfftm_deconvolve_transpose", "nfftm_convolve",
     "nfftm_convolve_transpose", "nfftm_convolve_typed", "nfftm_convolve_transpose_typed", "nfftm_fft", "nfftm_sizes", "nfftm_grid",
     "nfftm_timing", "nfftm_set_timing", "nfftm_profile", "nfftm_profile_read",
@@ -29,7 +29,7 @@ EXPORTS = (
     "nfftm_tile_shape", "nfftm_plan_set_tiles", "nfftm_batch_create", "nfftm_batch_info", "nfftm_batch_forward",
     "nfftm_batch_adjoint", "nfftm_batch_destroy", "nfftm_sdc", "nfftm_sdc_noalloc", "nfftm_toeplitz_create",
     "nfftm_toeplitz_kernel", "nfftm_toeplitz_apply", "nfftm_toeplitz_destroy",
-    "nfftm_plan_destroy", "nfftm_last_error", "nfftm_version",
+    "nfftm_plan_destroy", "nfftm_last_error", "nfftm_version", "nfftm_lds_issue_cycles",
 )
 
 _lib = None
@@ -52,6 +52,7 @@ def load():
     pi64 = ctypes.POINTER(ctypes.c_int64)
     pd = ctypes.POINTER(ctypes.c_double)
     L.nfftm_plan_create.argtypes = [ctypes.POINTER(vp), i, i, pi64, i64, pd, i, d, i, i, i, vp]
+    L.nfftm_plan_create_dims.argtypes = [ctypes.POINTER(vp), i, i, pi64, i, i, i64, pd, i, d, i, i, i, vp]
     L.nfftm_nodes.argtypes = [vp, pd, i64]
     L.nfftm_grad_nodes.argtypes = [vp, vp, vp, vp]
     L.nfftm_plan_tune.argtypes = [vp]
@@ -65,6 +66,7 @@ def load():
     L.nfftm_batch_destroy.argtypes = [vp]
     L.nfftm_batch_destroy.restype = None
     L.nfftm_sdc.argtypes = [vp, i, vp, i]
+    L.nfftm_lds_issue_cycles.argtypes = [pd, pd, ctypes.POINTER(i)]
     L.nfftm_sdc_noalloc.argtypes = [vp, i, vp, i]
     L.nfftm_toeplitz_create.argtypes = [ctypes.POINTER(vp), i, i, pi64, vp]
     L.nfftm_toeplitz_kernel.argtypes = [vp, vp, vp]

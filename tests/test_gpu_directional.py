@@ -70,3 +70,97 @@ def test_directional_equals_slices(N, dims, dtype):
     if orc is not None:
         assert rel(fhat, ofh) < 1e-12
         assert rel(g, og) < 1e-12
+
+
+STAGE_CASES = [((6, 16, 10), (1,)), ((12, 10, 8), (0, 1)), ((8, 10, 12), (1, 2))]
+
+
+@pytest.mark.parametrize("N,dims", STAGE_CASES, ids=[f"{'x'.join(map(str, c[0]))}-d{''.join(map(str, c[1]))}"
+                                                     for c in STAGE_CASES])
+def test_directional_stage_operators(N, dims):
+    """deconvolve!, convolve!, convolve_transpose!, deconvolve_transpose! and the
+    FFT of a directional plan on the whole arrays (src/directional.jl:1-156)
+    equal the slice-wise stage operators of the plan over N[dims] (FP64,
+    rel-l2 <= 1e-13)."""
+    print(f"directional stages N={N} dims={dims}", flush=True)
+    rng = np.random.default_rng([7, len(N), len(dims)])
+    J = 300
+    k = rng.random((len(dims), J)) - 0.5
+    cd = torch.complex128
+    p_dir = nfftm.plan_nfft(k, N, dims=dims, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=torch.float64)
+    Nd = tuple(N[a] for a in dims)
+    p = nfftm.plan_nfft(k, Nd, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=torch.float64)
+    G = p_dir.Ntilde
+    assert G == tuple(p.Ntilde[dims.index(a)] if a in dims else N[a] for a in range(len(N)))
+    f = torch.as_tensor(rng.standard_normal(N) + 1j * rng.standard_normal(N), dtype=cd, device="cuda")
+    fh = torch.as_tensor(rng.standard_normal(p_dir.NOut) + 1j * rng.standard_normal(p_dir.NOut), dtype=cd,
+                         device="cuda")
+    g = torch.zeros(G, dtype=cd, device="cuda")
+    p_dir.deconvolve_(f, g)
+    fh_d = torch.zeros(p_dir.NOut, dtype=cd, device="cuda")
+    p_dir.convolve_(g, fh_d)
+    g2 = torch.zeros(G, dtype=cd, device="cuda")
+    p_dir.convolve_transpose_(fh, g2)
+    f2 = torch.zeros(N, dtype=cd, device="cuda")
+    p_dir.deconvolve_transpose_(g2, f2)
+    g3 = g2.clone()
+    p_dir.fft_(g3, -1)
+    r_g, r_fh = torch.zeros_like(g), torch.zeros_like(fh_d)
+    r_g2, r_f2, r_g3 = torch.zeros_like(g2), torch.zeros_like(f2), torch.zeros_like(g3)
+    for idx in slices(N, dims):
+        oidx = idx[:dims[0]] + (slice(None),) + idx[dims[-1] + 1:]
+        gs = torch.zeros(p.Ntilde, dtype=cd, device="cuda")
+        p.deconvolve_(f[idx].contiguous(), gs)
+        r_g[idx] = gs
+        o = torch.zeros(J, dtype=cd, device="cuda")
+        p.convolve_(g[idx].contiguous(), o)
+        r_fh[oidx] = o
+        gs2 = torch.zeros(p.Ntilde, dtype=cd, device="cuda")
+        p.convolve_transpose_(fh[oidx].contiguous(), gs2)
+        r_g2[idx] = gs2
+        fs = torch.zeros(Nd, dtype=cd, device="cuda")
+        p.deconvolve_transpose_(g2[idx].contiguous(), fs)
+        r_f2[idx] = fs
+        gs3 = g2[idx].clone()  # a copy: the FFT is in place (g2[idx] may already be contiguous)
+        p.fft_(gs3, -1)
+        r_g3[idx] = gs3
+    torch.cuda.synchronize()
+    errs = {name: rel(a, b) for name, a, b in (("deconvolve", g, r_g), ("convolve", fh_d, r_fh),
+                                                ("convolve_transpose", g2, r_g2),
+                                                ("deconvolve_transpose", f2, r_f2), ("fft", g3, r_g3))}
+    print(errs, flush=True)
+    assert all(e < 1e-13 for e in errs.values()), errs
+
+
+def test_directional_nodes_copy_graphs_errors():
+    """nodes! on a directional plan equals a fresh plan, copy() transforms the
+    same, graph replay (three replays with other work between) equals eager
+    launches, and the operations it does not define raise."""
+    N, dims = (6, 32, 20), (1, 2)
+    rng = np.random.default_rng(41)
+    k1 = rng.random((2, 900)) - 0.5
+    k2 = rng.random((2, 700)) - 0.5
+    cd = torch.complex64
+    f = torch.as_tensor(rng.standard_normal(N) + 1j * rng.standard_normal(N), dtype=cd, device="cuda")
+    p = nfftm.plan_nfft(k1, N, dims=dims, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=torch.float32)
+    p.nodes_(k2)
+    q = nfftm.plan_nfft(k2, N, dims=dims, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=torch.float32)
+    y = p * f
+    assert y.shape == (6, 700) and rel(y, q * f) < 1e-6
+    c = p.copy()
+    assert isinstance(c, nfftm.DirectionalPlan) and rel(c * f, y) < 1e-6
+    x = p.H * y
+    pg = nfftm.plan_nfft(k2, N, dims=dims, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=torch.float32,
+                         graphs=True)
+    yg = torch.empty_like(y)
+    xg = torch.empty_like(x)
+    for _ in range(3):
+        pg.forward(f, yg)
+        pg.adjoint(yg, xg)
+        torch.cuda.synchronize()
+        _ = (f * 2).sum().item()
+        assert rel(yg, y) < 1e-6 and rel(xg, x) < 1e-6
+    with pytest.raises(nfftm.NFFTError):
+        p.grad_nodes(f, y)
+    with pytest.raises(nfftm.NFFTError):
+        nfftm.BatchPlan([p], [1])
