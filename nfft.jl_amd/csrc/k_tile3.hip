@@ -618,7 +618,11 @@ __global__ __launch_bounds__((lf::LineGeo<R, L>::NT)) void k_fold_fft3(KArgs a, 
   lf::load_tw<LG::NT>(tw, f.tw[2], L);
   const C* scr = (const C*)a.scratch;
   const int ntx = a.ntiles[0], nty = a.ntiles[1];
-  const int ox = blockIdx.x, y = blockIdx.y, oy = y / T, ly = y % T;
+  // workgroups of consecutive (ox, y) on one XCD, so that the halo lines
+  // neighbouring workgroups share stay in that XCD's L2 (gridDim.x =
+  // ntiles[0] L[1], a multiple of 8)
+  const int wg = xcd_item((int)gridDim.x);
+  const int ox = wg % ntx, y = wg / ntx, oy = y / T, ly = y % T;
   for (int i = threadIdx.x; i < 9 * NZ; i += LG::NT) {
     const int dxy = i / NZ, tz = i % NZ;
     const int tx = (ox + dxy % 3 - 1 + ntx) % ntx, ty = (oy + dxy / 3 - 1 + nty) % nty;
@@ -672,7 +676,7 @@ static hipError_t fold_fft3(const KArgs& a, const FArgs& f, hipStream_t s, void*
     return hipErrorNotSupported;
   } else {
     if (dry) return hipSuccess;
-    hipLaunchKernelGGL((k_fold_fft3<R, W, V, L>), dim3((unsigned)a.ntiles[0], (unsigned)f.L[1]),
+    hipLaunchKernelGGL((k_fold_fft3<R, W, V, L>), dim3((unsigned)(a.ntiles[0] * f.L[1])),
                        dim3(lf::LineGeo<R, L>::NT), 0, s, a, f, (typename cpx<R>::T*)grid);
     return hipGetLastError();
   }
