@@ -58,11 +58,39 @@ MEASURED_TRAFFIC = {("3d", "spreading"): (1075470013, None), ("3d", "interpolati
 STAGES = ("deconvolve", "fft_forward", "interpolation", "fold", "spreading", "fft_backward",
           "deconvolve_transpose", "node_gradient")
 
-# Measured issue cost of the LDS operations of the gridding taps on MI355X
-# (cycles per wave64 instruction per CU, microbenchmark in the library:
-# nfftm_lds_issue_cycles), and the clock they are counted at.
-CLOCK_HZ = 2.4e9
-N_CU = 256
+# Issue cost of the LDS instructions of the 3-D gridding taps, measured live
+# (nfftm_lds_issue_cycles: per-CU shader cycles of one wavefront instruction).
+LDS_OPS = {"ds_read_b64": 0, "ds_read_b128": 1, "ds_add_u64": 2, "ds_add_f64": 3}
+
+
+def lds_roofline(L, stage, ms, D, W, M, fp32):
+    """Secondary (LDS-issue) bound of the 3-D gridding kernels: the time the
+    taps' LDS instructions take at the measured per-CU issue cost, against
+    the kernel's measured time.  Spreading (k_spread3): a wavefront takes one
+    node, its lanes the W^2 footprint columns (ceil(W^2 / 64) instructions),
+    each adding W cells along z -- packed fixed point, one ds_add_u64 per cell
+    (FP32), or two ds_add_f64 (FP64).  Interpolation (k_interp3): 16 lanes per
+    node, each reading ceil(W^2 / 16) columns of W cells (ds_read_b64), 4
+    nodes per wavefront."""
+    if ms is None or D != 3 or stage not in ("spreading", "interpolation"):
+        return None
+    cyc = (ctypes.c_double * 4)()
+    hz = ctypes.c_double()
+    ncu = ctypes.c_int()
+    if L.nfftm_lds_issue_cycles(cyc, ctypes.byref(hz), ctypes.byref(ncu)) != 0:
+        return None
+    if stage == "interpolation":
+        op, per_node = "ds_read_b64", -(-W * W // 16) * W / 4.0
+    elif fp32:
+        op, per_node = "ds_add_u64", -(-W * W // 64) * W
+    else:
+        op, per_node = "ds_add_f64", 2 * -(-W * W // 64) * W
+    c = cyc[LDS_OPS[op]]
+    floor_s = M * per_node * c / (ncu.value * hz.value)
+    return {"bound": "lds", "kernel": stage, "instruction": op, "instructions_per_node": per_node,
+            "cycles_per_instruction": c, "clock_hz": hz.value, "cus": ncu.value,
+            "floor_ms": floor_s * 1e3, "avg_ms": ms, "frac": floor_s * 1e3 / ms,
+            "issue_cycles_measured": {k: cyc[i] for k, i in LDS_OPS.items()}}
 
 
 def source_digest():
@@ -222,6 +250,43 @@ def run_batch(args, torch, nfftm, dev):
         del b, plans, fs, fh, ys, xs
         torch.cuda.empty_cache()
     Mtot = sum(k.shape[1] * n for _, k, n in specs)
+    # algorithmic bytes of one forward + adjoint of every slot (FP32, s_c = 8)
+    sc = 8
+    bytes_step = 0
+    for N, k, n in specs:
+        P, Q, M = N[0] * N[1], 4 * N[0] * N[1], k.shape[1]
+        bytes_step += n * ((P * sc + 4 * Q * sc + M * (2 * 8 + sc)) + (2 * P * sc + 3 * Q * sc + M * (2 * 8 + sc)))
+    achieved = bytes_step / res["f32"]["t_batch"]
+    cpu = None
+    if not args.no_cpu:
+        # a bounded sample of the same workload: the shared radial trajectory
+        # (one of the 32 coils) and the first 8 uneven plans, FP64 blocked
+        # LINEAR oracle plans on the host cores, forward + adjoint
+        import oracle as O
+        threads = O.lib().orc_num_threads()
+        log(f"bench: CPU baseline (oracle restatement, {threads} threads)")
+        sample = [specs[-1]] + specs[:8]
+        t_cpu, m_cpu = 0.0, 0
+        for N, k, _ in sample:
+            op = O.OraclePlan(k, N, 4, 2.0, O.LINEAR, True, workers=threads)
+            rng_c = np.random.default_rng(11)
+            fc = rng_c.standard_normal(N) + 1j * rng_c.standard_normal(N)
+            fhc = rng_c.standard_normal(k.shape[1]) + 1j * rng_c.standard_normal(k.shape[1])
+            op.forward(fc)
+            op.adjoint(fhc)
+            best = float("inf")
+            for _ in range(3):
+                t0 = time.perf_counter()
+                op.forward(fc)
+                op.adjoint(fhc)
+                best = min(best, time.perf_counter() - t0)
+            t_cpu += best
+            m_cpu += k.shape[1]
+        cpu = {"value": m_cpu / t_cpu, "unit": "NU points/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(),
+               "sample": f"{len(sample)} of the batch's plans (the radial trajectory once and the first 8 "
+                         f"uneven plans, {m_cpu} nodes), FP64 blocked LINEAR restatement, one plan at a time, "
+                         "min of 3 forward+adjoint runs each, FFT by scipy.fft"}
     e32 = max(float(np.linalg.norm(a.numpy() - c.numpy()) / np.linalg.norm(c.numpy()))
               for a, c in zip(res["f32"]["y"] + res["f32"]["x"], res["f64"]["y"] + res["f64"]["x"]))
     out = {
@@ -251,8 +316,10 @@ def run_batch(args, torch, nfftm, dev):
         "graph": res["f32"]["graph"],
         "setup_s": {"f32": res["f32"]["setup_s"], "f64": res["f64"]["setup_s"]},
         "tiles_f32": res["f32"]["tiles"],
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "kernel": "whole batch step (every stage of all 96 slots)",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_step": bytes_step},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
@@ -409,7 +476,9 @@ def main():
     log("bench: creating FP32 and FP64 plans")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    p32 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float32)
+    # the small 2-D workload is launch-bound: its transforms replay HIP graphs
+    graphs = args.workload == "2d-spiral"
+    p32 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float32, graphs=graphs)
     torch.cuda.synchronize()
     pre32 = time.perf_counter() - t0
     # nodes! on the same nodes (device binning, records): the per-update cost
@@ -461,7 +530,7 @@ def main():
     # FP64 plan on the same workload
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    p64 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float64)
+    p64 = nfftm.plan_nfft(k, N, m=m, sigma=sigma, precompute=pre, dtype=torch.float64, graphs=graphs)
     torch.cuda.synchronize()
     pre64 = time.perf_counter() - t0
     if args.tune:
@@ -597,8 +666,11 @@ def main():
                "sample": f"full workload ({desc}; FP64 blocked {pre_name} plan), min of {reps} "
                          "forward+adjoint runs, FFT by scipy.fft"}
 
+    metric = {"3d": "NU points/s fwd+adj 3D 128³ M=2²¹: FP32 vs FP64 GPU at rel-l2≤1e-5; HBM %",
+              "1d": "NU points/s fwd+adj 1D N=M=2²⁴ edge-clustered, m=6 σ=1.25 TENSOR (FP32)",
+              "2d-spiral": "NU points/s fwd+adj+node gradient 2D 256² spiral M=65536 (FP32)"}[args.workload]
     out = {
-        "metric": "NU points/s fwd+adj 3D 128³ M=2²¹: FP32 vs FP64 GPU at rel-l2≤1e-5; HBM %",
+        "metric": metric,
         "value": M / t32,
         "unit": "NU points/s",
         "n_gpus": 1,
@@ -611,7 +683,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": desc, "N": list(N), "M": M, "m": m, "sigma": sigma,
-                   "precompute": pre_name},
+                   "precompute": pre_name, "hip_graphs": graphs},
         "hbm_frac_total": B_total / t32 / HBM_PEAK,
         "fp64": {"value": M / t64, "ms_per_step": t64 * 1e3},
         "f32_over_f64": t64 / t32,
@@ -621,6 +693,9 @@ def main():
         "stage_ms": stage_ms,
         "tiles": {"f32": list(p32.tile_shape()), "f64": list(p64.tile_shape())},
         "roofline": roofline,
+        "roofline_lds": lds_roofline(L, dom, stage_ms[dom], D, 2 * m, M, True),
+        "roofline_lds_interpolation": lds_roofline(L, "interpolation", stage_ms["interpolation"], D, 2 * m, M,
+                                                   True),
         "cpu_baseline": cpu,
     }
     if sparse is not None:

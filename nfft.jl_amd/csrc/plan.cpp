@@ -812,7 +812,18 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
     ntile *= a.ntiles[d];
   }
   a.ntile = (int)ntile;
-  a.CH = D == 3 ? ITEM3 : D == 2 ? 2048 : 4096;
+  // nodes per work item: 3-D, the fixed item of the tile kernels; D <= 2,
+  // twice the mean nodes per tile (256 .. 2048 / 4096), so that a crowded tile
+  // (the centre of a spiral or radial trajectory) spreads over several
+  // workgroups instead of serialising the launch
+  if (D == 3) {
+    a.CH = ITEM3;
+  } else {
+    const i64 mean = (M + ntile - 1) / ntile, cap = D == 2 ? 2048 : 4096;
+    i64 ch = 256;
+    while (ch < 2 * mean && ch < cap) ch *= 2;
+    a.CH = (int)ch;
+  }
   a.key_bits = 1;
   while (((i64)1 << a.key_bits) < ntile) ++a.key_bits;
   DevBuf<double> kd;

@@ -5,7 +5,8 @@ The device result (nfftm_binning: sorted permutation and work items) must
 equal, entry for entry, a numpy restatement: shift into [0, 1) as
 shiftNodes! (src/utils.jl:32-44), tile of floor(k * Ntilde) per dimension,
 a stable sort by tile (so a tile's nodes keep the caller's order), and work
-items of at most CH nodes per tile.  Cases: 1-D, 2-D and 3-D in both
+items of at most CH nodes per tile (3-D: 1024; D <= 2: twice the mean
+nodes per tile, a power of two in 256 .. 2048 / 4096).  Cases: 1-D, 2-D and 3-D in both
 precisions, nodes on the range ends (-1/2, +1/2, -0.0, a negative node that
 shifts to exactly 1), tiles crowded beyond one work item, empty tiles,
 nodes! with a different M, and a plan on the node-parallel path.  The
@@ -23,11 +24,23 @@ pytestmark = pytest.mark.gpu
 EPS = 2.220446049250313e-16
 
 
+def item_size(D, M, Nt, T):
+    """Most nodes of one work item: 3-D, 1024; D <= 2, twice the mean nodes
+    per tile, a power of two in 256 .. 2048 (2-D) / 4096 (1-D)."""
+    if D == 3:
+        return 1024
+    ntile = int(np.prod([-(-Nt[d] // T[d]) for d in range(D)]))
+    mean, cap, ch = -(-M // ntile), {1: 4096, 2: 2048}[D], 256
+    while ch < 2 * mean and ch < cap:
+        ch *= 2
+    return ch
+
+
 def expected(k, Nt, T):
     """Python axis order in, Julia tile numbering out (dimension 1 fastest)."""
     D, M = k.shape
-    CH = {1: 4096, 2: 2048, 3: 1024}[D]
     kj, Ntj, Tj = k[::-1], Nt[::-1], T[::-1]
+    CH = item_size(D, M, Ntj, Tj)
     key = np.zeros(M, dtype=np.int64)
     stride = 1
     for d in range(D):
@@ -74,7 +87,9 @@ def test_device_binning_equals_restatement(dtype, case):
     p = nfftm.plan_nfft(k, N, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=dtype)
     items = check(p, k)
     assert len(items) > 0 and (np.diff(items[:, 0]) >= 0).all()
-    assert (items[:, 2] - items[:, 1]).max() == {1: 4096, 2: 2048, 3: 1024}[D]  # crowded: a full item
+    T, _ = p.tile_shape()
+    ch = item_size(D, k.shape[1], p.Ntilde[::-1], T[::-1])
+    assert (items[:, 2] - items[:, 1]).max() == ch  # crowded: a full item
     k2 = rng.random((D, 777)) - 0.5  # nodes! with another M
     p.nodes_(k2)
     check(p, k2)
