@@ -222,7 +222,10 @@ void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
   const int Nx = a.Nt[0], Ny = a.Nt[1], Nz = a.Nt[2];
   const long long sy = a.gstr[1], sz = a.gstr[2];
 
-  // stage the padded tile
+  // the first batch's records are loaded while the padded tile is staged,
+  // each later batch's during the taps of the one before
+  RecStage<G, NT> rs;
+  rs.load(a, it.y, min(G::NB, it.z - it.y));
   stage3<G, NT, STAGE_RB<R>>(grid, tile, t0[0] - a.m + 1, t0[1] - a.m + 1, t0[2] - a.m + 1, Nx, Ny, Nz, sy, sz,
                              (unsigned long long)a.gstr[2] * (unsigned long long)Nz <= 0xFFFFFFFFull);
 
@@ -232,8 +235,9 @@ void k_interp3(KArgs a, const typename cpx<R>::T* __restrict__ grid,
   for (int b0 = it.y; b0 < it.z; b0 += G::NB) {
     const int nb = min(G::NB, it.z - b0);
     __syncthreads();
-    phase_a<R, 3, W, NT, false>(a, nullptr, b0, nb, lrec, nullptr, (R)1);
+    rs.store(lrec, nb);
     __syncthreads();
+    if (b0 + G::NB < it.z) rs.load(a, b0 + G::NB, min(G::NB, it.z - b0 - G::NB));
     for (int n0 = 4 * wave; n0 < nb; n0 += 4 * NW) {
       const int n = n0 + quarter;
       const bool live = n < nb;
