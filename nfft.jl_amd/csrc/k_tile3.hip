@@ -307,6 +307,8 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
     hre[h] = v.x;
     him[h] = v.y;
   }
+  // clear the tile while the gathers are in flight
+  for (int i = threadIdx.x; i < (int)(G::TILE_BYTES / 16); i += NT) ((uint4*)tile)[i] = make_uint4(0, 0, 0, 0);
 
   // FP32: fixed-point scale s = 2^(30 - e), B = sum_j |f_j|_inf * wmaxD < 2^e
   R fscale = 1, inv = 1;
@@ -334,7 +336,6 @@ void k_spread3(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
       inv = (R)NAN;
     }
   }
-  for (int i = threadIdx.x; i < (int)(G::TILE_BYTES / 16); i += NT) ((uint4*)tile)[i] = make_uint4(0, 0, 0, 0);
 
   const R sc = fscale * (R)a.oscale;
   for (int b0 = it.y; b0 < it.z; b0 += G::NB) {
