@@ -10,10 +10,15 @@
 // values and derivatives are FP64 (analytic Kaiser-Bessel derivative; in the
 // POLYNOMIAL and TENSOR modes the derivative of the fitted polynomial, so the
 // gradient is that of the operator the plan applies); the gather accumulates
-// in the data precision.  One thread per node; the D-dimensional footprint is
-// reduced row by row: for every outer index (l_1 .. l_{D-1}) the two dot
-// products of the 2m-point row with w_0 and dw_0 are formed once and then
-// weighted by the outer products.
+// in the data precision.
+//
+// D <= 3 (k_grad_lanes): a group of G lanes per node (G = 2m rounded up to a
+// power of two), one lane per tap along dimension 1.  Lane q evaluates tap q
+// of the window in every dimension; the outer taps reach the group by
+// shuffles, each footprint row is read by the group as one run of 2m
+// consecutive cells, and the lanes' partial sums meet in a butterfly.
+// D > 3 (k_grad_nodes): one thread per node, the footprint reduced row by row
+// over an odometer of the outer indices.
 #include "launch.hpp"
 
 namespace nfftm {
@@ -49,40 +54,44 @@ __device__ __forceinline__ void kb_value_deriv(double u, double m, double b, dou
   *dv = -u * q / M_PI;
 }
 
-// Window values and node derivatives of one node along dimension d, with the
-// plan's power-of-two window normalisation; returns the wrapped cell.
+// Window value and node derivative of tap l (0 <= l < 2m) at window argument
+// t = k Ntilde_d - off, with the plan's power-of-two window normalisation.
+__device__ __forceinline__ void tap_weight(const KArgs& a, double Ntd, double t, int l, double* w, double* dw) {
+  if (a.mode == WIN_POLY) {
+    const int nc = 2 * a.m + 1;
+    const double* cf = a.polyd + l * nc;
+    const double x = t - (double)a.m + 0.5;
+    double s = cf[nc - 1], ds = 0.0;
+    for (int i = nc - 2; i >= 0; --i) {
+      ds = fma(x, ds, s);
+      s = fma(x, s, cf[i]);
+    }
+    *w = s;
+    *dw = Ntd * ds;
+  } else {
+    double v, dv;
+    kb_value_deriv(t - (double)l, (double)a.m, a.b, &v, &dv);
+    *w = v * a.wscale;
+    *dw = Ntd * dv * a.wscale;
+  }
+}
+
+// Window argument of one node along dimension d; returns the wrapped cell.
+__device__ __forceinline__ int grad_cell(const KArgs& a, int d, double ksv, double* t) {
+  const double kscale = scaled_node(ksv, a.Ntd[d]);
+  const long long c = (long long)kscale;
+  *t = kscale - (double)(c - a.m + 1);
+  return (int)(c >= a.Nt[d] ? c - a.Nt[d] : c);
+}
+
+// Window values and node derivatives of all 2m taps along dimension d.
 template <int W>
 __device__ __forceinline__ int grad_weights(const KArgs& a, int d, double ksv, double* w, double* dw) {
-  const double Ntd = a.Ntd[d];
-  const double kscale = scaled_node(ksv, Ntd);
-  const long long c = (long long)kscale;
-  const long long off = c - a.m + 1;
-  const double t = kscale - (double)off;
-  if (a.mode == WIN_POLY) {
-    const double* P = a.polyd;
-    const int nc = 2 * a.m + 1;
-    const double x = t - (double)a.m + 0.5;
+  double t;
+  const int cw = grad_cell(a, d, ksv, &t);
 #pragma unroll
-    for (int l = 0; l < W; ++l) {
-      const double* cf = P + l * nc;
-      double s = cf[nc - 1], ds = 0.0;
-      for (int i = nc - 2; i >= 0; --i) {
-        ds = fma(x, ds, s);
-        s = fma(x, s, cf[i]);
-      }
-      w[l] = s;
-      dw[l] = Ntd * ds;
-    }
-  } else {
-#pragma unroll
-    for (int l = 0; l < W; ++l) {
-      double v, dv;
-      kb_value_deriv(t - (double)l, (double)a.m, a.b, &v, &dv);
-      w[l] = v * a.wscale;
-      dw[l] = Ntd * dv * a.wscale;
-    }
-  }
-  return (int)(c >= a.Nt[d] ? c - a.Nt[d] : c);
+  for (int l = 0; l < W; ++l) tap_weight(a, a.Ntd[d], t, l, &w[l], &dw[l]);
+  return cw;
 }
 
 // Row of 2m grid points along dimension 1 at offset ofs: dot products with
@@ -101,29 +110,110 @@ __device__ __forceinline__ void grad_row(const C* __restrict__ grid, long long o
   }
 }
 
-// v[i] for a runtime i of a register array (a select chain: no private
-// memory for the array).
-template <int W, typename R>
-__device__ __forceinline__ R pick(const R* v, int i) {
-  R r = v[0];
-#pragma unroll
-  for (int l = 1; l < W; ++l) r = i == l ? v[l] : r;
-  return r;
+// Lanes per node of k_grad_lanes: 2m rounded up to a power of two.
+template <int W>
+constexpr int grad_group() {
+  return W <= 2 ? 2 : W <= 4 ? 4 : W <= 8 ? 8 : 16;
 }
 
 template <typename R, int W, int DD>
+__global__ __launch_bounds__(256) void k_grad_lanes(KArgs a, const typename cpx<R>::T* __restrict__ grid,
+                                                    const typename cpx<R>::T* __restrict__ ybar,
+                                                    double* __restrict__ kbar) {
+  using C = typename cpx<R>::T;
+  constexpr int G = grad_group<W>();
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int s = (int)(t / G), q = (int)(t % G);
+  // the G lanes of a node are consecutive and aligned: they leave together
+  if (s >= a.M) return;
+  const bool tap = q < W;
+  R w[DD], dw[DD];
+  long long base[DD];
+#pragma unroll
+  for (int d = 0; d < DD; ++d) {
+    double td, wd = 0.0, dwd = 0.0;
+    const int cw = grad_cell(a, d, a.ks[(size_t)d * a.M + s], &td);
+    if (tap) tap_weight(a, a.Ntd[d], td, q, &wd, &dwd);
+    w[d] = (R)wd;
+    dw[d] = (R)dwd;
+    base[d] = (long long)cw - a.m + 1;
+  }
+  // cell of this lane's tap along dimension 1 (lanes past 2m read tap 0 with
+  // zero weights)
+  const long long xi = wrapidx(base[0] + (tap ? q : 0), a.Nt[0]);
+  R gr[DD], gi[DD];
+#pragma unroll
+  for (int d = 0; d < DD; ++d) gr[d] = gi[d] = 0;
+  if constexpr (DD == 1) {
+    const C v = grid[xi];
+    gr[0] = dw[0] * v.x;
+    gi[0] = dw[0] * v.y;
+  } else if constexpr (DD == 2) {
+    // row l_1: d/dk_0 weight dw_0 w_1, d/dk_1 weight w_0 dw_1
+#pragma unroll
+    for (int ly = 0; ly < W; ++ly) {
+      const R w1 = __shfl(w[1], ly, G), dw1 = __shfl(dw[1], ly, G);
+      const C v = grid[wrapidx(base[1] + ly, a.Nt[1]) * a.gstr[1] + xi];
+      const R c0 = dw[0] * w1, c1 = w[0] * dw1;
+      gr[0] = fma(c0, v.x, gr[0]);
+      gi[0] = fma(c0, v.y, gi[0]);
+      gr[1] = fma(c1, v.x, gr[1]);
+      gi[1] = fma(c1, v.y, gi[1]);
+    }
+  } else {
+    // plane l_2, row l_1: weights dw_0 w_1 w_2, w_0 dw_1 w_2, w_0 w_1 dw_2
+    R w1s[W], dw1s[W];
+#pragma unroll
+    for (int ly = 0; ly < W; ++ly) {
+      w1s[ly] = __shfl(w[1], ly, G);
+      dw1s[ly] = __shfl(dw[1], ly, G);
+    }
+#pragma unroll 1
+    for (int lz = 0; lz < W; ++lz) {
+      const R w2 = __shfl(w[2], lz, G), dw2 = __shfl(dw[2], lz, G);
+      const R p0 = dw[0] * w2, p1 = w[0] * w2, p2 = w[0] * dw2;
+      const long long oz = wrapidx(base[2] + lz, a.Nt[2]) * a.gstr[2] + xi;
+#pragma unroll
+      for (int ly = 0; ly < W; ++ly) {
+        const C v = grid[oz + wrapidx(base[1] + ly, a.Nt[1]) * a.gstr[1]];
+        const R c0 = p0 * w1s[ly], c1 = p1 * dw1s[ly], c2 = p2 * w1s[ly];
+        gr[0] = fma(c0, v.x, gr[0]);
+        gi[0] = fma(c0, v.y, gi[0]);
+        gr[1] = fma(c1, v.x, gr[1]);
+        gi[1] = fma(c1, v.y, gi[1]);
+        gr[2] = fma(c2, v.x, gr[2]);
+        gi[2] = fma(c2, v.y, gi[2]);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = G / 2; o >= 1; o >>= 1)
+#pragma unroll
+    for (int d = 0; d < DD; ++d) {
+      gr[d] += __shfl_xor(gr[d], o, G);
+      gi[d] += __shfl_xor(gi[d], o, G);
+    }
+  if (q == 0) {
+    const int jj = a.perm[s];
+    const C y = ybar[jj];
+#pragma unroll
+    for (int d = 0; d < DD; ++d)
+      kbar[(size_t)d + (size_t)DD * jj] = (double)y.x * (double)gr[d] + (double)y.y * (double)gi[d];
+  }
+}
+
+// Any D (used for D > 3): one thread per node.
+template <typename R, int W>
 __global__ __launch_bounds__(256) void k_grad_nodes(KArgs a, const typename cpx<R>::T* __restrict__ grid,
                                                     const typename cpx<R>::T* __restrict__ ybar,
                                                     double* __restrict__ kbar) {
   using C = typename cpx<R>::T;
-  constexpr int DM = DD ? DD : MAXD;
-  const int D = DD ? DD : a.D;
+  constexpr int DM = MAXD;
+  const int D = a.D;
   const int s = blockIdx.x * 256 + threadIdx.x;
   if (s >= a.M) return;
   R w[DM][W], dw[DM][W];
   long long base[DM];
-  // compile-time dimension loop for D <= 3 (register arrays), runtime beyond
-#pragma unroll(DD ? DD : 1)
   for (int d = 0; d < D; ++d) {
     double wd[W], dwd[W];
     const int cw = grad_weights<W>(a, d, a.ks[(size_t)d * a.M + s], wd, dwd);
@@ -140,51 +230,8 @@ __global__ __launch_bounds__(256) void k_grad_nodes(KArgs a, const typename cpx<
   R gr[DM], gi[DM];
 #pragma unroll
   for (int d = 0; d < DM; ++d) gr[d] = gi[d] = 0;
-  if constexpr (DD == 1) {
-    R sr, si;
-    grad_row<R, W>(grid, 0, x0, w[0], dw[0], sr, si, gr[0], gi[0]);
-  } else if constexpr (DD == 2) {
-    // rows l_1: d/dk_0 takes (dw_0 . row) w_1, d/dk_1 takes (w_0 . row) dw_1
-#pragma unroll 1
-    for (int ly = 0; ly < W; ++ly) {
-      R sr, si, tr, ti;
-      grad_row<R, W>(grid, wrapidx(base[1] + ly, a.Nt[1]) * a.gstr[1], x0, w[0], dw[0], sr, si, tr, ti);
-      const R w1 = pick<W>(w[1], ly), dw1 = pick<W>(dw[1], ly);
-      gr[0] = fma(tr, w1, gr[0]);
-      gi[0] = fma(ti, w1, gi[0]);
-      gr[1] = fma(sr, dw1, gr[1]);
-      gi[1] = fma(si, dw1, gi[1]);
-    }
-  } else if constexpr (DD == 3) {
-    // planes l_2 of rows l_1: plane sums A = sum (dw_0 . row) w_1,
-    // B = sum (w_0 . row) dw_1, E = sum (w_0 . row) w_1, then weighted by
-    // w_2 (A, B) and dw_2 (E)
-#pragma unroll 1
-    for (int lz = 0; lz < W; ++lz) {
-      const long long oz = wrapidx(base[2] + lz, a.Nt[2]) * a.gstr[2];
-      R Ar = 0, Ai = 0, Br = 0, Bi = 0, Er = 0, Ei = 0;
-#pragma unroll 1
-      for (int ly = 0; ly < W; ++ly) {
-        R sr, si, tr, ti;
-        grad_row<R, W>(grid, oz + wrapidx(base[1] + ly, a.Nt[1]) * a.gstr[1], x0, w[0], dw[0], sr, si, tr, ti);
-        const R w1 = pick<W>(w[1], ly), dw1 = pick<W>(dw[1], ly);
-        Ar = fma(tr, w1, Ar);
-        Ai = fma(ti, w1, Ai);
-        Br = fma(sr, dw1, Br);
-        Bi = fma(si, dw1, Bi);
-        Er = fma(sr, w1, Er);
-        Ei = fma(si, w1, Ei);
-      }
-      const R w2 = pick<W>(w[2], lz), dw2 = pick<W>(dw[2], lz);
-      gr[0] = fma(Ar, w2, gr[0]);
-      gi[0] = fma(Ai, w2, gi[0]);
-      gr[1] = fma(Br, w2, gr[1]);
-      gi[1] = fma(Bi, w2, gi[1]);
-      gr[2] = fma(Er, dw2, gr[2]);
-      gi[2] = fma(Ei, dw2, gi[2]);
-    }
-  } else {
-    // any D: odometer over the outer indices
+  {
+    // odometer over the outer indices
     int l[DM];
     for (int d = 0; d < DM; ++d) l[d] = 0;
     while (true) {
@@ -221,19 +268,22 @@ __global__ __launch_bounds__(256) void k_grad_nodes(KArgs a, const typename cpx<
 template <typename R, int W>
 static hipError_t grad_d(const KArgs& a, hipStream_t s, const void* g, const void* ybar, double* kbar) {
   using C = typename cpx<R>::T;
-  const dim3 grid((a.M + 255) / 256), blk(256);
-  switch (a.D) {
+  constexpr long long G = grad_group<W>();
+  const dim3 lanes((unsigned)(((long long)a.M * G + 255) / 256)), blk(256);
+  // one thread per node where a lane per tap would pass 2^31 threads
+  switch ((long long)a.M * G < (1LL << 31) ? a.D : 0) {
     case 1:
-      hipLaunchKernelGGL((k_grad_nodes<R, W, 1>), grid, blk, 0, s, a, (const C*)g, (const C*)ybar, kbar);
+      hipLaunchKernelGGL((k_grad_lanes<R, W, 1>), lanes, blk, 0, s, a, (const C*)g, (const C*)ybar, kbar);
       break;
     case 2:
-      hipLaunchKernelGGL((k_grad_nodes<R, W, 2>), grid, blk, 0, s, a, (const C*)g, (const C*)ybar, kbar);
+      hipLaunchKernelGGL((k_grad_lanes<R, W, 2>), lanes, blk, 0, s, a, (const C*)g, (const C*)ybar, kbar);
       break;
     case 3:
-      hipLaunchKernelGGL((k_grad_nodes<R, W, 3>), grid, blk, 0, s, a, (const C*)g, (const C*)ybar, kbar);
+      hipLaunchKernelGGL((k_grad_lanes<R, W, 3>), lanes, blk, 0, s, a, (const C*)g, (const C*)ybar, kbar);
       break;
     default:
-      hipLaunchKernelGGL((k_grad_nodes<R, W, 0>), grid, blk, 0, s, a, (const C*)g, (const C*)ybar, kbar);
+      hipLaunchKernelGGL((k_grad_nodes<R, W>), dim3((a.M + 255) / 256), blk, 0, s, a, (const C*)g,
+                         (const C*)ybar, kbar);
   }
   return hipGetLastError();
 }
