@@ -476,6 +476,43 @@ hipError_t launch_copy_strided(bool dbl, const void* src, long long ss, void* ds
   return hipGetLastError();
 }
 
+// Inverse of the binning permutation: iperm[perm[j]] = j.
+__global__ __launch_bounds__(256) void k_inv_perm(const int* __restrict__ perm, int* __restrict__ iperm, long long n) {
+  for (long long j = (long long)blockIdx.x * 256 + threadIdx.x; j < n; j += (long long)gridDim.x * 256)
+    iperm[perm[j]] = (int)j;
+}
+
+hipError_t launch_inv_perm(const int* perm, int* iperm, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_inv_perm, dim3(blocks), dim3(256), 0, s, perm, iperm, n);
+  return hipGetLastError();
+}
+
+// Node values from sorted order to the caller's order: out[i * os] =
+// in[iperm[i]].  Coalesced stores, gathered loads: on large node sets this is
+// cheaper than storing each result at its caller position from the sorted
+// kernel (8-byte stores at random positions cost a partial-line write each).
+template <typename C>
+__global__ __launch_bounds__(256) void k_unsort(const C* __restrict__ in, const int* __restrict__ iperm,
+                                                C* __restrict__ out, long long os, long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    out[i * os] = in[iperm[i]];
+}
+
+hipError_t launch_unsort(bool dbl, const void* in, const int* iperm, void* out, long long os, long long n,
+                         hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+  if (dbl)
+    hipLaunchKernelGGL(k_unsort<double2>, dim3(blocks), dim3(256), 0, s, (const double2*)in, iperm, (double2*)out,
+                       os, n);
+  else
+    hipLaunchKernelGGL(k_unsort<float2>, dim3(blocks), dim3(256), 0, s, (const float2*)in, iperm, (float2*)out, os,
+                       n);
+  return hipGetLastError();
+}
+
 size_t tile_lds_bytes(int D, bool dbl, int W, int tileElems) {
   int NB;
   if (dbl) NB = D == 1 ? Batch<1, double>::NB : D == 2 ? Batch<2, double>::NB : Batch<3, double>::NB;

@@ -104,6 +104,8 @@ struct KArgs {
   // nodes
   const double* ks;   // ks[d * M + j], shifted into [0, 1)
   const int* perm;    // sorted position -> caller's node index
+  void* fsort;        // interpolation results in sorted order (D = 1 polynomial
+                      // kernels; null: stored at the caller's positions)
   const int4* items;  // {tile, begin, end, 0}
   int nitems;
   const char* rec;    // node records in sorted order (NodeRec layout), tiled plans
@@ -186,6 +188,21 @@ __device__ __forceinline__ R window_exact(const KArgs& a, R u) {
   }
 }
 
+// The 2m fitted window polynomials at x = t - m + 1/2 (evalpoly per tap,
+// src/precomputation.jl:203-218): column l of P holds the 2m + 1 coefficients
+// of tap l, lowest degree first; Horner in the data precision.
+template <typename R, int W, typename PT = const R*>
+__device__ __forceinline__ void poly_taps(PT P, R x, R* w) {
+  constexpr int NC = W + 1;
+#pragma unroll
+  for (int l = 0; l < W; ++l) {
+    R s = P[l * NC + NC - 1];
+#pragma unroll
+    for (int i = NC - 2; i >= 0; --i) s = fma(x, s, P[l * NC + i]);
+    w[l] = s;
+  }
+}
+
 template <typename R, int W>
 __device__ __forceinline__ int node_weights(const KArgs& a, int d, double ksv, R* w) {
   const double kscale = scaled_node(ksv, a.Ntd[d]);
@@ -206,16 +223,7 @@ __device__ __forceinline__ int node_weights(const KArgs& a, int d, double ksv, R
       w[l] = y1 + alpha * (y2 - y1);
     }
   } else if (a.mode == WIN_POLY) {
-    const R* P = (const R*)a.poly;
-    const int nc = 2 * a.m + 1;
-    const R x = (R)(t - (double)a.m + 1.0 - 0.5);
-#pragma unroll
-    for (int l = 0; l < W; ++l) {
-      const R* cf = P + l * nc;
-      R s = cf[nc - 1];
-      for (int i = nc - 2; i >= 0; --i) s = fma(x, s, cf[i]);
-      w[l] = s;
-    }
+    poly_taps<R, W>((const R*)a.poly, (R)(t - (double)a.m + 1.0 - 0.5), w);
   } else {
 #pragma unroll
     for (int l = 0; l < W; ++l) w[l] = window_exact<R, W>(a, (R)(t - (double)l)) * (R)a.wscale;

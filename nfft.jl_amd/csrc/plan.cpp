@@ -443,6 +443,13 @@ struct nfftm_plan_s {
   DevBuf<int4> d_items;
   DevBuf<int> d_tstart;
   DevBuf<char> d_scratch, d_rec;
+  // D = 1 polynomial-window plans: the inverse binning permutation and the
+  // sorted-order interpolation results it maps back to the caller's order
+  DevBuf<int> d_iperm;
+  DevBuf<char> d_fsort;
+  bool unsort() const {
+    return tiled && D == 1 && precompute != NFFTM_LINEAR && precompute != NFFTM_FULL && d_iperm.p;
+  }
   // node-side staging vector (M values of the plan's complex type) of the
   // typed stage operators, allocated at their first call
   DevBuf<char> d_tmpM;
@@ -578,6 +585,7 @@ struct nfftm_plan_s {
     a.oscale = 1.0;
     a.ks = d_ks.p;
     a.perm = d_perm.p;
+    a.fsort = nullptr;
     a.items = d_items.p;
     a.nitems = nitems;
     a.rec = d_rec.p;
@@ -886,6 +894,13 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
     const size_t rb = (size_t)node_rec_bytes(D, p.dbl() ? sizeof(double) : sizeof(float), p.W);
     HIPCHK(p.d_rec.alloc(std::max<size_t>(1, (size_t)M) * rb));
     HIPCHK(launch_node_records(D, p.dbl(), p.W, p.kargs(), p.d_rec.p, p.stream));
+    p.d_iperm.reset();
+    p.d_fsort.reset();
+    if (D == 1 && p.precompute != NFFTM_LINEAR && p.precompute != NFFTM_FULL && M > 0) {
+      HIPCHK(p.d_fsort.alloc((size_t)M * p.celem()));
+      HIPCHK(p.d_iperm.alloc((size_t)M));
+      HIPCHK(launch_inv_perm(p.d_perm.p, p.d_iperm.p, M, p.stream));
+    }
   }
   return NFFTM_OK;
 }
@@ -975,7 +990,13 @@ int run_conv(nfftm_plan_s& p, bool spread, const void* in, void* out, bool stand
   }
   {
     ProfScope ps(p, spread ? 4 : 2);
-    if (p.tiled)
+    if (p.tiled && !spread && p.unsort()) {
+      // interpolation results in sorted order, then gathered into the
+      // caller's order
+      a.fsort = p.d_fsort.p;
+      e = launch_tile(p.D, false, p.dbl(), p.W, p.NT, a, p.lds, p.stream, in, nullptr);
+      if (e == hipSuccess) e = launch_unsort(p.dbl(), p.d_fsort.p, p.d_iperm.p, out, a.fst, p.M, p.stream);
+    } else if (p.tiled)
       e = launch_tile(p.D, spread, p.dbl(), p.W, p.NT, a, p.lds, p.stream, in, spread ? nullptr : out);
     else if (p.sparse)
       e = launch_sparse(spread, p.dbl(), a, p.sparseK, p.d_bcols.p, p.d_bvals.p, p.stream, in, out);
