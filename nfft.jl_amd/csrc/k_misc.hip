@@ -489,27 +489,30 @@ hipError_t launch_inv_perm(const int* perm, int* iperm, long long n, hipStream_t
   return hipGetLastError();
 }
 
-// Node values from sorted order to the caller's order: out[i * os] =
-// in[iperm[i]].  Coalesced stores, gathered loads: on large node sets this is
-// cheaper than storing each result at its caller position from the sorted
-// kernel (8-byte stores at random positions cost a partial-line write each).
+// Node values between sorted and caller order: out[i * os] = in[idx[i] * is]
+// (idx = iperm: results of the sorted interpolation back to the caller's
+// order).  Coalesced stores, gathered loads: on node sets larger than the
+// caches this is cheaper than the sorted kernel storing at the caller's
+// positions (8-byte stores at random positions cost a partial-line write
+// each; measured on 2^24 nodes: 337 us of stores against ~180 us here).
 template <typename C>
-__global__ __launch_bounds__(256) void k_unsort(const C* __restrict__ in, const int* __restrict__ iperm,
-                                                C* __restrict__ out, long long os, long long n) {
+__global__ __launch_bounds__(256) void k_gather_nodes(const C* __restrict__ in, long long is,
+                                                      const int* __restrict__ idx, C* __restrict__ out,
+                                                      long long os, long long n) {
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    out[i * os] = in[iperm[i]];
+    out[i * os] = in[(long long)idx[i] * is];
 }
 
-hipError_t launch_unsort(bool dbl, const void* in, const int* iperm, void* out, long long os, long long n,
-                         hipStream_t s) {
+hipError_t launch_gather_nodes(bool dbl, const void* in, long long is, const int* idx, void* out, long long os,
+                               long long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
   if (dbl)
-    hipLaunchKernelGGL(k_unsort<double2>, dim3(blocks), dim3(256), 0, s, (const double2*)in, iperm, (double2*)out,
-                       os, n);
+    hipLaunchKernelGGL(k_gather_nodes<double2>, dim3(blocks), dim3(256), 0, s, (const double2*)in, is, idx,
+                       (double2*)out, os, n);
   else
-    hipLaunchKernelGGL(k_unsort<float2>, dim3(blocks), dim3(256), 0, s, (const float2*)in, iperm, (float2*)out, os,
-                       n);
+    hipLaunchKernelGGL(k_gather_nodes<float2>, dim3(blocks), dim3(256), 0, s, (const float2*)in, is, idx,
+                       (float2*)out, os, n);
   return hipGetLastError();
 }
 

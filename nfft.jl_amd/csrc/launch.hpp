@@ -74,11 +74,11 @@ hipError_t launch_crop_corner(bool dbl, const void* x, void* y, const ToepArgs& 
 hipError_t launch_cmul(bool dbl, void* x, const void* lam, long long n, hipStream_t s);
 hipError_t launch_shift_sign(bool dbl, void* x, const ToepArgs& a, hipStream_t s);
 
-// iperm[perm[j]] = j, and out[i * os] = in[iperm[i]] (results of the sorted
-// kernels back to the caller's node order).
+// iperm[perm[j]] = j, and out[i * os] = in[idx[i] * is] (node values between
+// the caller's and the sorted order).
 hipError_t launch_inv_perm(const int* perm, int* iperm, long long n, hipStream_t s);
-hipError_t launch_unsort(bool dbl, const void* in, const int* iperm, void* out, long long os, long long n,
-                         hipStream_t s);
+hipError_t launch_gather_nodes(bool dbl, const void* in, long long is, const int* idx, void* out, long long os,
+                               long long n, hipStream_t s);
 
 // LDS bytes of a tile kernel.
 size_t tile_lds_bytes(int D, bool dbl, int W, int tileElems);
