@@ -30,10 +30,8 @@ __device__ __forceinline__ double shifted(double x) {
   return x;
 }
 
-__global__ __launch_bounds__(BT) void k_bin_keys(BinArgs a, const double* __restrict__ k, int* __restrict__ key,
-                                                 int* __restrict__ idx, int* __restrict__ cnt) {
-  const long long j = (long long)blockIdx.x * BT + threadIdx.x;
-  if (j >= a.M) return;
+// Tile of node j (k in the caller's layout, D values per node).
+__device__ __forceinline__ int node_tile(const BinArgs& a, const double* __restrict__ k, long long j) {
   long long t = 0, str = 1;
   for (int d = 0; d < a.D; ++d) {
     const double kscale = shifted(k[d + (long long)a.D * j]) * (double)a.Nt[d];
@@ -42,9 +40,38 @@ __global__ __launch_bounds__(BT) void k_bin_keys(BinArgs a, const double* __rest
     t += (c / a.T[d]) * str;
     str *= a.ntiles[d];
   }
-  key[j] = (int)t;
+  return (int)t;
+}
+
+__global__ __launch_bounds__(BT) void k_bin_keys(BinArgs a, const double* __restrict__ k, int* __restrict__ key,
+                                                 int* __restrict__ idx, int* __restrict__ cnt) {
+  const long long j = (long long)blockIdx.x * BT + threadIdx.x;
+  if (j >= a.M) return;
+  const int t = node_tile(a, k, j);
+  key[j] = t;
   idx[j] = (int)j;
   atomicAdd(&cnt[t], 1);
+}
+
+// The same with the histogram of each workgroup's nodes in LDS first (at most
+// HIST tiles): crowded tiles (thousands of nodes in one tile) then cost one
+// global atomic per workgroup instead of one per node.
+constexpr int HIST = 16384;
+constexpr int HIST_BLOCKS = 512;
+__global__ __launch_bounds__(BT) void k_bin_keys_lds(BinArgs a, const double* __restrict__ k, int* __restrict__ key,
+                                                     int* __restrict__ idx, int* __restrict__ cnt) {
+  __shared__ int hist[HIST];
+  for (int i = threadIdx.x; i < a.ntile; i += BT) hist[i] = 0;
+  __syncthreads();
+  for (long long j = (long long)blockIdx.x * BT + threadIdx.x; j < a.M; j += (long long)gridDim.x * BT) {
+    const int t = node_tile(a, k, j);
+    key[j] = t;
+    idx[j] = (int)j;
+    atomicAdd(&hist[t], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.ntile; i += BT)
+    if (hist[i]) atomicAdd(&cnt[i], hist[i]);
 }
 
 // One workgroup: off = exclusive scan of cnt (ntile + 1 entries), ioff =
@@ -140,7 +167,12 @@ size_t bin_sort_temp_bytes(long long M, int key_bits) {
 hipError_t launch_bin_keys(const BinArgs& a, const double* k, int* key, int* idx, int* cnt, hipStream_t s) {
   hipError_t e = launch_fill0(cnt, sizeof(int) * ((size_t)a.ntile + 1), s);
   if (e != hipSuccess || a.M == 0) return e;
-  hipLaunchKernelGGL(k_bin_keys, dim3(blocks(a.M)), dim3(BT), 0, s, a, k, key, idx, cnt);
+  if (a.ntile <= HIST) {
+    const unsigned nb = (unsigned)std::min<long long>(HIST_BLOCKS, (a.M + BT - 1) / BT);
+    hipLaunchKernelGGL(k_bin_keys_lds, dim3(nb), dim3(BT), 0, s, a, k, key, idx, cnt);
+  } else {
+    hipLaunchKernelGGL(k_bin_keys, dim3(blocks(a.M)), dim3(BT), 0, s, a, k, key, idx, cnt);
+  }
   return hipGetLastError();
 }
 
