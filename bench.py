@@ -47,11 +47,13 @@ F32_TOL = 1e-5  # rel-l2 bar of every FP32 result (against the FP64 plan and the
 # (scaled by a 256 MiB elementwise copy in the same run: factors 2.0 / 1.0);
 # profiles/nfft3d_128cubed_M2p21_hbm_traffic.md holds the passes and the
 # command.  Keyed by (workload, stage) for the default FP32 plan of that
-# workload, together with the digest of the kernel sources they were
-# measured on (source_digest()): when the sources change, the figure is
-# reported as stale (traffic null) until it is measured again.
-MEASURED_TRAFFIC = {("3d", "spreading"): (1087150269, "4a1295a0972b21a7"),
-                    ("3d", "interpolation"): (644896908, "4a1295a0972b21a7")}
+# workload, together with the digest of the sources the 3-D gridding kernels
+# are built from (source_digest(TRAFFIC_SOURCES)) when they were measured:
+# when those sources change, the figure is reported as stale (traffic null)
+# until it is measured again.
+TRAFFIC_SOURCES = ("k_tile3.hip", "kernels.hpp", "launch.hpp", "fft_line.hpp")
+MEASURED_TRAFFIC = {("3d", "spreading"): (1087150269, "a1686306c6acf3c5"),
+                    ("3d", "interpolation"): (644896908, "a1686306c6acf3c5")}
 
 # Stage order of nfftm_profile_read.  "fold": the per-item padded tiles summed
 # into the grid; on 3-D power-of-two grids with one work item per tile it is
@@ -94,13 +96,14 @@ def lds_roofline(L, stage, ms, D, W, M, fp32):
             "issue_cycles_measured": {k: cyc[i] for k, i in LDS_OPS.items()}}
 
 
-def source_digest():
-    """sha256 (16 hex digits) of the HIP sources the kernels are built from."""
+def source_digest(names=None):
+    """sha256 (16 hex digits) of the HIP sources the kernels are built from
+    (all of them, or the files in names)."""
     import hashlib
     h = hashlib.sha256()
     d = os.path.join(REPO, "nfft.jl_amd", "csrc")
     for name in sorted(os.listdir(d)):
-        if name.endswith((".hip", ".hpp", ".inc")):
+        if (name in names) if names else name.endswith((".hip", ".hpp", ".inc")):
             with open(os.path.join(d, name), "rb") as fh:
                 h.update(name.encode() + b"\0" + fh.read())
     return h.hexdigest()[:16]
@@ -635,13 +638,15 @@ def main():
     dom = max((s for s in mine if stage_ms[s]), key=lambda s: stage_ms[s])
     achieved = sb[dom] / (stage_ms[dom] * 1e-3)
     digest = source_digest()
+    tdigest = source_digest(TRAFFIC_SOURCES)
     meas = None if args.tune else MEASURED_TRAFFIC.get((args.workload, dom))
-    traffic = meas[0] if meas is not None and meas[1] == digest else None
+    traffic = meas[0] if meas is not None and meas[1] == tdigest else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "bytes_per_launch": sb[dom], "avg_ms": stage_ms[dom],
                 "traffic_over_algorithmic": None if traffic is None else traffic / sb[dom],
-                "traffic_source_digest": None if meas is None else meas[1], "source_digest": digest}
+                "traffic_source_digest": None if meas is None else meas[1], "gridding_source_digest": tdigest,
+                "source_digest": digest}
     B_total = (P * sc + 4 * Q * sc + M * (D * 8 + sc)) + (2 * P * sc + 3 * Q * sc + M * (D * 8 + sc))
 
     cpu = None
