@@ -23,12 +23,15 @@ namespace nfftm {
 constexpr int NT1 = 256;
 constexpr int STAGE1 = 8;  // tile cells each thread loads before writing them
 
-// The 2m window values of sorted node j and its wrapped cell.  The
+// The 2m window values of sorted node j and its wrapped cell.  FP64: the
 // coefficient pointer is made opaque per node so that the compiler loads the
 // (2m + 1) x 2m coefficients into scalar registers tap by tap inside the node
 // loop instead of keeping all of them live across it (at 2m = 12 that is 156
 // values, more than the scalar register file, and they would be spilled to
-// vector lanes).
+// vector lanes).  FP32 up to 2m = 12 keeps half of them (at most 78), which
+// fit, live across the loop: a scalar load in the loop would wait (the wait
+// counter is shared with LDS operations) for the previous node's LDS atomics
+// as well.
 template <typename R, int W>
 __device__ __forceinline__ int node_taps1(const KArgs& a, double ksv, R* w) {
   const double kscale = scaled_node(ksv, a.Ntd[0]);
@@ -36,8 +39,27 @@ __device__ __forceinline__ int node_taps1(const KArgs& a, double ksv, R* w) {
   const double t = kscale - (double)(c - a.m + 1);
   using CR = const __attribute__((address_space(4))) R;  // scalar loads
   CR* P = (CR*)a.poly;
-  asm volatile("" : "+s"(P));
-  poly_taps<R, W, CR*>(P, (R)(t - (double)a.m + 1.0 - 0.5), w);
+  const R x = (R)(t - (double)a.m + 1.0 - 0.5);
+  if constexpr (sizeof(R) == 4 && W <= 12) {
+    // the window is symmetric and the fit's sample points are too, so tap
+    // 2m - 1 - l is tap l's polynomial at -x: the m (2m + 1) coefficients
+    // of taps 0 .. m - 1 stay in scalar registers across the node loop
+    constexpr int NC = W + 1;
+#pragma unroll
+    for (int l = 0; l < W / 2; ++l) {
+      R s = P[l * NC + NC - 1], u = s;
+#pragma unroll
+      for (int i = NC - 2; i >= 0; --i) {
+        s = fma(x, s, P[l * NC + i]);
+        u = fma(-x, u, P[l * NC + i]);
+      }
+      w[l] = s;
+      w[W - 1 - l] = u;
+    }
+  } else {
+    asm volatile("" : "+s"(P));
+    poly_taps<R, W, CR*>(P, x, w);
+  }
   return (int)(c >= a.Nt[0] ? c - a.Nt[0] : c);
 }
 
