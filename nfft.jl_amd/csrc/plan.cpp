@@ -800,6 +800,14 @@ int check_nodes(const double* k, i64 n) {
 // on the host (checkNodes, src/utils.jl:46-55, before any device work), then
 // on the device (k_bin.hip): shift into [0, 1) (src/utils.jl:32-44), bin by
 // tile (src/precomputation.jl:459-520 at tile granularity) with a stable
+// Node records (NodeRec layout) of the plan's sorted nodes.
+int build_records(nfftm_plan_s& p) {
+  const size_t rb = (size_t)node_rec_bytes(p.D, p.dbl() ? sizeof(double) : sizeof(float), p.W);
+  HIPCHK(p.d_rec.alloc(std::max<size_t>(1, (size_t)p.M) * rb));
+  HIPCHK(launch_node_records(p.D, p.dbl(), p.W, p.kargs(), p.d_rec.p, p.stream));
+  return NFFTM_OK;
+}
+
 // sort, split each tile's nodes into work items, gather the shifted nodes in
 // tile order.
 int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
@@ -893,10 +901,11 @@ int set_nodes(nfftm_plan_s& p, const double* k, i64 M) {
   }
   if (p.tiled) {
     HIPCHK(p.d_scratch.alloc(std::max<size_t>(1, (size_t)nitems) * p.slot_cells() * p.celem()));
-    // node records in tile order, built on the device
-    const size_t rb = (size_t)node_rec_bytes(D, p.dbl() ? sizeof(double) : sizeof(float), p.W);
-    HIPCHK(p.d_rec.alloc(std::max<size_t>(1, (size_t)M) * rb));
-    HIPCHK(launch_node_records(D, p.dbl(), p.W, p.kargs(), p.d_rec.p, p.stream));
+    // node records in tile order, built on the device (1-D polynomial-window
+    // plans evaluate the window in their kernels; a batch that takes such a
+    // plan builds its records then)
+    if (p.sorted_io()) p.d_rec.reset();
+    else if (int rc = build_records(p)) return rc;
     p.d_iperm.reset();
     p.d_fsort.reset();
     if (p.sorted_io() && M > 0) {
@@ -1420,6 +1429,10 @@ int batch_build(nfftm_batch_s& b, const nfftm_plan* plans, const int* ntrans, in
     for (auto* p : b.plans)
       if (int rc = tune_plan(*p, 512, nullptr)) return rc;
   for (auto* p : b.plans) {
+    // the batch kernels read node records, which 1-D polynomial-window plans
+    // do not keep for themselves
+    if (!p->d_rec.p)
+      if (int rc = build_records(*p)) return rc;
     HIPCHK(hipStreamSynchronize(p->stream));
     if (p->NT != 256 && p->NT != 512) return fail(NFFTM_ERR_UNSUPPORTED, "batch kernels take 256 or 512 threads");
     b.gens.push_back(p->gen);
