@@ -552,25 +552,25 @@ __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<
 // [tT - m + 1, tT + T + m - 1] modulo Ntilde; with that range no longer than
 // Ntilde and 2T >= 2m - 1 (checked by the host) a point has at most 3
 // covering (tile, offset) pairs per dimension.
-__device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
-
+// Covering pairs in the order of the images y = x - Nt, x, x + Nt and
+// ascending tile: tile 0 (y = x - Nt, its left halo wrapped), tiles t - 1,
+// t, t + 1 of x's own tile t (y = x), the last two tiles (y = x + Nt, a
+// right halo wrapped).  With T >= m no other tile can cover x, so one
+// division per point suffices.
 __device__ __forceinline__ int cover_pairs(const KArgs& a, int d, int x, int* tl, int* pl) {
   const int T = a.T[d], P = a.Pd[d], Nt = a.Nt[d], nt = a.ntiles[d], m1 = a.m - 1;
+  const int t = x / T;
+  const int cu[6] = {0, t - 1, t, t + 1, nt - 2, nt - 1};
+  const int cy[6] = {x - Nt, x, x, x, x + Nt, x + Nt};
   int n = 0;
 #pragma unroll
-  for (int k = -1; k <= 1; ++k) {
-    const int y = x + k * Nt;
-    int tlo = -floordiv(-(y - T - m1), T);  // ceil((y - T - m + 1) / T)
-    int thi = floordiv(y + m1, T);
-    tlo = max(tlo, 0);
-    thi = min(thi, nt - 1);
-    for (int t = tlo; t <= thi && n < 3; ++t) {
-      const int p = y - t * T + m1;
-      if (p >= 0 && p < P) {
-        tl[n] = t;
-        pl[n] = p;
-        ++n;
-      }
+  for (int q = 0; q < 6; ++q) {
+    const int u = cu[q];
+    const int p = cy[q] - u * T + m1;
+    if (u >= 0 && u < nt && p >= 0 && p < P && n < 3) {
+      tl[n] = u;
+      pl[n] = p;
+      ++n;
     }
   }
   return n;
