@@ -23,7 +23,8 @@ namespace nfftm {
 constexpr int NT1 = 256;
 constexpr int STAGE1 = 8;  // tile cells each thread loads before writing them
 
-// The 2m window values of sorted node j and its wrapped cell.  FP64: the
+// The 2m window values of the node at shifted coordinate ksv and its wrapped
+// cell.  FP64: the
 // coefficient pointer is made opaque per node so that the compiler loads the
 // (2m + 1) x 2m coefficients into scalar registers tap by tap inside the node
 // loop instead of keeping all of them live across it (at 2m = 12 that is 156
