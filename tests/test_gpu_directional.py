@@ -36,22 +36,19 @@ CASES = [((8, 12), (0,)), ((8, 12), (1,)), ((10, 8, 6), (0,)), ((10, 8, 6), (1,)
          ((10, 8, 12), (0, 1)), ((10, 8, 12), (1, 2)), ((4, 8, 6, 10), (1, 2, 3))]
 
 
-@pytest.mark.parametrize("N,dims", CASES, ids=[f"{'x'.join(map(str, c[0]))}-d{''.join(map(str, c[1]))}" for c in CASES])
-@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
-def test_directional_equals_slices(N, dims, dtype):
-    print(f"directional N={N} dims={dims} {dtype}", flush=True)
+def check_directional(N, dims, dtype, pre=nfftm.LINEAR, opre=O.LINEAR, J=None):
     rng = np.random.default_rng([len(N), sum(N), len(dims)])
-    J = int(np.prod(N))
+    J = int(np.prod(N)) if J is None else J
     k = rng.random((len(dims), J)) - 0.5
     cd = torch.complex128 if dtype == torch.float64 else torch.complex64
-    p_dir = nfftm.plan_nfft(k, N, dims=dims, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=dtype)
+    p_dir = nfftm.plan_nfft(k, N, dims=dims, m=4, sigma=2.0, precompute=pre, dtype=dtype)
     Nd = tuple(N[a] for a in dims)
     assert p_dir.size_out() == N[:dims[0]] + (J,) + N[dims[-1] + 1:]
     f = torch.as_tensor(rng.standard_normal(N) + 1j * rng.standard_normal(N), dtype=cd, device="cuda")
     fhat = p_dir * f
     g = p_dir.H * fhat
-    p = nfftm.plan_nfft(k, Nd, m=4, sigma=2.0, precompute=nfftm.LINEAR, dtype=dtype)
-    orc = O.OraclePlan(k, Nd, m=4, sigma=2.0, precompute=O.LINEAR) if dtype == torch.float64 else None
+    p = nfftm.plan_nfft(k, Nd, m=4, sigma=2.0, precompute=pre, dtype=dtype)
+    orc = O.OraclePlan(k, Nd, m=4, sigma=2.0, precompute=opre) if dtype == torch.float64 else None
     ref_fhat = torch.empty_like(fhat)
     ref_g = torch.empty_like(g)
     ofh, og = np.empty(fhat.shape, complex), np.empty(g.shape, complex)
@@ -70,6 +67,30 @@ def test_directional_equals_slices(N, dims, dtype):
     if orc is not None:
         assert rel(fhat, ofh) < 1e-12
         assert rel(g, og) < 1e-12
+    return p
+
+
+@pytest.mark.parametrize("N,dims", CASES, ids=[f"{'x'.join(map(str, c[0]))}-d{''.join(map(str, c[1]))}" for c in CASES])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_directional_equals_slices(N, dims, dtype):
+    print(f"directional N={N} dims={dims} {dtype}", flush=True)
+    check_directional(N, dims, dtype)
+
+
+# 1-D directions long enough for the tiled kernels: with the fitted-polynomial
+# window they evaluate the window in the kernel and store their results in
+# sorted order, gathered back along the strided (axis 0) or contiguous axis.
+POLY_CASES = [((512, 6), (0,)), ((6, 512), (1,)), ((3, 400, 4), (1,))]
+
+
+@pytest.mark.parametrize("N,dims", POLY_CASES, ids=[f"{'x'.join(map(str, c[0]))}-d{''.join(map(str, c[1]))}"
+                                                    for c in POLY_CASES])
+@pytest.mark.parametrize("pre", ["POLYNOMIAL", "TENSOR"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_directional_polynomial_1d(N, dims, pre, dtype):
+    print(f"directional {pre} N={N} dims={dims} {dtype}", flush=True)
+    p = check_directional(N, dims, dtype, getattr(nfftm, pre), getattr(O, pre), J=3000)
+    assert p.tile_shape()[0][0] < p.Ntilde[0]  # the tiled path ran
 
 
 STAGE_CASES = [((6, 16, 10), (1,)), ((12, 10, 8), (0, 1)), ((8, 10, 12), (1, 2))]
