@@ -152,6 +152,15 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat_dev, void* f_dev);
  * plan's grid. */
 int nfftm_grad_nodes(nfftm_plan p, const void* f_dev, const void* ybar_dev, double* kbar_dev);
 
+/* The same node gradient from the transformed grid g = FFT(deconvolve(f)) of
+ * the forward transform, as the plan's grid (nfftm_grid) holds it after
+ * nfftm_forward: only the window-derivative gather runs.  No counterpart in
+ * the reference; for pullbacks that keep their forward pass's grid (an
+ * rrule's pullback, a loss evaluated and differentiated in one step).
+ * g: prod(Ntilde) complex (may be the plan's own grid); ybar, kbar as in
+ * nfftm_grad_nodes. */
+int nfftm_grad_nodes_grid(nfftm_plan p, const void* g_dev, const void* ybar_dev, double* kbar_dev);
+
 /* deconvolve!(p, f, g): src/deconvolution.jl:2-45, including the
  * fill!(tmpVec, 0) of src/implementation.jl:159.  g: prod(Ntilde) complex. */
 int nfftm_deconvolve(nfftm_plan p, const void* f_dev, void* g_dev);

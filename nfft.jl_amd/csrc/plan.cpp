@@ -1834,19 +1834,33 @@ int nfftm_adjoint(nfftm_plan p, const void* fhat, void* f) {
   });
 }
 
-int nfftm_grad_nodes(nfftm_plan p, const void* f, const void* ybar, double* kbar) {
+static int grad_check(nfftm_plan p, const void* in, i64 nin, const void* ybar, double* kbar) {
   g_err.clear();
   if (!p) return fail(NFFTM_ERR_ARG, "plan is NULL");
   if (p->window != NFFTM_KAISER_BESSEL && (p->precompute == NFFTM_LINEAR || p->precompute == NFFTM_FULL))
     return fail(NFFTM_ERR_UNSUPPORTED,
                 "node gradient of window %d needs precompute TENSOR or POLYNOMIAL (the analytic window "
                 "derivative is built for Kaiser-Bessel)", p->window);
-  if ((!f && p->P) || (!ybar && p->M) || (!kbar && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
+  if ((!in && nin) || (!ybar && p->M) || (!kbar && p->M)) return fail(NFFTM_ERR_ARG, "NULL data pointer");
   if (p->dir()) return fail(NFFTM_ERR_UNSUPPORTED, "node gradient of a directional plan");
+  return NFFTM_OK;
+}
+
+int nfftm_grad_nodes(nfftm_plan p, const void* f, const void* ybar, double* kbar) {
+  if (int rc = grad_check(p, f, p ? p->P : 0, ybar, kbar)) return rc;
   if (int rc = forward_to_grid(*p, f, [] { return (int)NFFTM_OK; })) return rc;
   if (p->M) {
     ProfScope ps(*p, 7);
     HIPCHK(launch_grad_nodes(p->dbl(), p->W, p->kargs(), p->stream, p->d_grid.p, ybar, kbar));
+  }
+  return NFFTM_OK;
+}
+
+int nfftm_grad_nodes_grid(nfftm_plan p, const void* g, const void* ybar, double* kbar) {
+  if (int rc = grad_check(p, g, p ? p->Q : 0, ybar, kbar)) return rc;
+  if (p->M) {
+    ProfScope ps(*p, 7);
+    HIPCHK(launch_grad_nodes(p->dbl(), p->W, p->kargs(), p->stream, g, ybar, kbar));
   }
   return NFFTM_OK;
 }

@@ -345,14 +345,33 @@ function grad_nodes(p::MI355XNFFTPlan{T,D}, f::ROCArray, ȳ::ROCArray) where {T,
   return Array(kbar)
 end
 
+"""
+    grad_nodes_grid(p, g, ȳ) -> k̄::Matrix{Float64}
+
+The node gradient of `grad_nodes` from the transformed grid g of the forward
+transform (`p.tmpVec` right after `p * f`, or a copy of it): only the
+window-derivative gather runs.
+"""
+function grad_nodes_grid(p::MI355XNFFTPlan{T,D}, g::ROCArray{Complex{T}}, ȳ::ROCArray) where {T,D}
+  size(g) == p.Ñ || throw(DimensionMismatch("size(g) = $(size(g)) ≠ Ñ = $(p.Ñ)"))
+  (eltype(ȳ) == Complex{T} && length(ȳ) == p.J) ||
+    throw(DimensionMismatch("ȳ must be a Complex{$T} vector of length J = $(p.J)"))
+  kbar = ROCArray{Float64}(undef, D, p.J)
+  check(ccall((:nfftm_grad_nodes_grid, libnfftm), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+              p.handle, devptr(g), devptr(ȳ), devptr(kbar)))
+  return Array(kbar)
+end
+
 # AbstractNFFTs/ext/AbstractNFFTsChainRulesCoreExt.jl:26-45 returns NoTangent()
-# for the nodes; this rule returns their gradient as well.
+# for the nodes; this rule returns their gradient as well, from the forward
+# pass's transformed grid.
 function ChainRulesCore.rrule(::typeof(AbstractNFFTs.nfft), k::Matrix{T}, x::ROCArray) where {T}
   p = AbstractNFFTs.plan_nfft(MI355XBackend(), ROCArray, k, size(x))
   y = p * x
+  g = copy(p.tmpVec)
   function nfft_pullback(ȳ)
     ȳ = ROCArray(ChainRulesCore.unthunk(ȳ))
-    k̄ = T.(grad_nodes(p, x, ȳ))
+    k̄ = T.(grad_nodes_grid(p, g, ȳ))
     x̄ = adjoint(p) * ȳ
     return ChainRulesCore.NoTangent(), k̄, x̄
   end

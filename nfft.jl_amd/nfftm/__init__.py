@@ -453,6 +453,22 @@ class NFFTPlan:
                                           ctypes.c_void_p(kb.data_ptr())))
         return kb.t().flip(0)
 
+    def grad_nodes_grid(self, g, ybar):
+        """The node gradient of grad_nodes from the transformed grid g of the
+        forward transform (the plan's tmpVec right after ``p * f``, or a copy
+        of it) instead of f: only the window-derivative gather runs
+        (nfftm_grad_nodes_grid)."""
+        if self.directional:
+            raise NFFTError("node gradient of a directional plan")
+        g = self._prep(g, self.Ntilde, "g")
+        ybar = self._prep(ybar, (self.J,), "ybar")
+        kb = torch.empty((self.J, self.D), dtype=torch.float64, device=_device())
+        self._lib.nfftm_set_stream(self._handle, _stream())
+        _check(self._lib.nfftm_grad_nodes_grid(self._handle, ctypes.c_void_p(g.data_ptr()),
+                                               ctypes.c_void_p(ybar.data_ptr()),
+                                               ctypes.c_void_p(kb.data_ptr())))
+        return kb.t().flip(0)
+
     def tune(self):
         """Autotune the tile edge and workgroup size of the gridding kernels on
         this plan's nodes (nfftm_plan_tune); returns the chosen shape."""

@@ -25,7 +25,8 @@ EXPORTS = (
     "nfftm_deconvolve", "nfftm_deconvolve_transpose", "nfftm_convolve",
     "nfftm_convolve_transpose", "nfftm_convolve_typed", "nfftm_convolve_transpose_typed", "nfftm_fft", "nfftm_sizes", "nfftm_grid",
     "nfftm_timing", "nfftm_set_timing", "nfftm_profile", "nfftm_profile_read",
-    "nfftm_set_stream", "nfftm_plan_copy", "nfftm_binning", "nfftm_grad_nodes", "nfftm_plan_tune",
+    "nfftm_set_stream", "nfftm_plan_copy", "nfftm_binning", "nfftm_grad_nodes", "nfftm_grad_nodes_grid",
+    "nfftm_plan_tune",
     "nfftm_tile_shape", "nfftm_plan_set_tiles", "nfftm_batch_create", "nfftm_batch_info", "nfftm_batch_forward",
     "nfftm_batch_adjoint", "nfftm_batch_destroy", "nfftm_sdc", "nfftm_sdc_noalloc", "nfftm_toeplitz_create",
     "nfftm_toeplitz_kernel", "nfftm_toeplitz_apply", "nfftm_toeplitz_destroy",
@@ -55,6 +56,7 @@ def load():
     L.nfftm_plan_create_dims.argtypes = [ctypes.POINTER(vp), i, i, pi64, i, i, i64, pd, i, d, i, i, i, vp]
     L.nfftm_nodes.argtypes = [vp, pd, i64]
     L.nfftm_grad_nodes.argtypes = [vp, vp, vp, vp]
+    L.nfftm_grad_nodes_grid.argtypes = [vp, vp, vp, vp]
     L.nfftm_plan_tune.argtypes = [vp]
     L.nfftm_binning.argtypes = [vp, ctypes.POINTER(i), vp, vp]
     L.nfftm_plan_set_tiles.argtypes = [vp, i, i]

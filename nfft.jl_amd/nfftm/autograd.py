@@ -24,9 +24,12 @@ def rrule_nfft(k, x, plan=None, **kw):
     if plan is None:
         plan = NFFTPlan(k, tuple(x.shape), **kw)
     y = plan * x
+    # the forward pass's transformed grid, kept for the node cotangent (the
+    # pullback then runs only the window-derivative gather)
+    g = plan.tmpVec.clone()
 
     def nfft_pullback(ybar):
-        kbar = plan.grad_nodes(x, ybar)
+        kbar = plan.grad_nodes_grid(g, ybar)
         xbar = plan.H * ybar
         return kbar, xbar
 
@@ -49,19 +52,21 @@ class NFFTFunction(torch.autograd.Function):
                             **(kwargs or {}))
         ctx.plan = plan
         ctx.k_meta = (k.device, k.dtype)
-        ctx.save_for_backward(x)
-        return plan * x
+        ctx.x_dtype = x.dtype
+        y = plan * x
+        # the transformed grid of this forward pass, for the node cotangent
+        ctx.grid = plan.tmpVec.clone() if ctx.needs_input_grad[0] else None
+        return y
 
     @staticmethod
     def backward(ctx, ybar):
-        (x,) = ctx.saved_tensors
         ybar = ybar.contiguous()
         kbar = xbar = None
         if ctx.needs_input_grad[0]:
             dev, dt = ctx.k_meta
-            kbar = ctx.plan.grad_nodes(x, ybar).to(device=dev, dtype=dt)
+            kbar = ctx.plan.grad_nodes_grid(ctx.grid, ybar).to(device=dev, dtype=dt)
         if ctx.needs_input_grad[1]:
-            xbar = (ctx.plan.H * ybar).to(x.dtype)
+            xbar = (ctx.plan.H * ybar).to(ctx.x_dtype)
         return kbar, xbar, None, None
 
 

@@ -1,5 +1,5 @@
-"""GPU node gradient (nfftm_grad_nodes through the C ABI) and the
-differentiation rules built on it.
+"""GPU node gradient (nfftm_grad_nodes and nfftm_grad_nodes_grid through the
+C ABI) and the differentiation rules built on it.
 
 The reference's rrules return NoTangent() for the nodes
 (AbstractNFFTs/ext/AbstractNFFTsChainRulesCoreExt.jl:14-45); the library adds
@@ -148,6 +148,35 @@ def test_autograd_function_and_rrule():
     xn = x.detach().cpu().numpy()
     assert rel(x.grad, O.ndft_adjoint(kn, N, 2 * r.cpu().numpy())) < 1e-6
     assert rel(k.grad, O.ndft_grad(kn, xn, 2 * r.cpu().numpy())) < 1e-5
+
+
+GRID_CASES = [((64, 48), nfftm.POLYNOMIAL), ((300,), nfftm.LINEAR), ((12, 10, 16), nfftm.TENSOR),
+              ((30, 20), nfftm.LINEAR)]
+
+
+@pytest.mark.parametrize("N,pre", GRID_CASES, ids=lambda v: str(v))
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_grad_from_forward_grid(N, pre, dtype):
+    """nfftm_grad_nodes_grid on the grid a forward transform leaves (the
+    plan's own grid, and a copy of it after other transforms) equals
+    nfftm_grad_nodes recomputing that grid, bit for bit: the same gather on
+    the same grid.  Power-of-two and other grids (pruned passes and rocFFT)."""
+    banner(f"grad from grid N={N} pre={pre} {dtype}")
+    rng = np.random.default_rng([len(N), sum(N), pre])
+    M = 700
+    k = rng.random((len(N), M)) - 0.5
+    cd = torch.complex128 if dtype == torch.float64 else torch.complex64
+    p = nfftm.plan_nfft(k, N, m=4, sigma=2.0, precompute=pre, dtype=dtype)
+    f = torch.as_tensor(crandn(rng, N), dtype=cd, device="cuda")
+    ybar = torch.as_tensor(crandn(rng, M), dtype=cd, device="cuda")
+    ref = p.grad_nodes(f, ybar)
+    _ = p * f
+    g = p.tmpVec.clone()
+    assert torch.equal(p.grad_nodes_grid(p.tmpVec, ybar), ref)
+    _ = p.H * ybar  # the plan's grid now holds other values
+    assert torch.equal(p.grad_nodes_grid(g, ybar), ref)
+    with pytest.raises(nfftm.DimensionMismatch):
+        p.grad_nodes_grid(g.reshape(-1)[:-1], ybar)
 
 
 def test_grad_edge_cases():
