@@ -10,7 +10,7 @@ the dynamic range of fHat, so the cases here are the hard ones:
 * Gaussian clusters on a 128^3 grid that fill many tiles beyond 1024 nodes;
 * data of magnitude 1e-30 (the scale exponent clamp; the result must scale
   linearly) and non-finite data (the output must not be finite garbage).
-Bars: FP32 vs FP64 plan rel-l2 <= 1e-5 (the north-star FP32 bar); FP64 vs
+Bars: FP32 vs FP64 plan rel-l2 <= 1e-5 (the FP32 parity bar); FP64 vs
 the oracle <= 1e-12.
 """
 import numpy as np
