@@ -127,13 +127,39 @@ __global__ __launch_bounds__(256) void k_grad_lanes(KArgs a, const typename cpx<
   // the G lanes of a node are consecutive and aligned: they leave together
   if (s >= a.M) return;
   const bool tap = q < W;
+  // the cotangent of the node, loaded ahead of the gather
+  const int jj = a.perm[s];
+  const C y = ybar[jj];
+  // fitted-polynomial modes: this lane's tap has the same coefficients in
+  // every dimension, loaded once (the arithmetic of tap_weight)
+  const bool poly = a.mode == WIN_POLY;
+  double cq[W + 1];
+  if (poly) {
+    const double* cf = a.polyd + (tap ? q : 0) * (W + 1);
+#pragma unroll
+    for (int i = 0; i <= W; ++i) cq[i] = cf[i];
+  }
   R w[DD], dw[DD];
   long long base[DD];
 #pragma unroll
   for (int d = 0; d < DD; ++d) {
     double td, wd = 0.0, dwd = 0.0;
     const int cw = grad_cell(a, d, a.ks[(size_t)d * a.M + s], &td);
-    if (tap) tap_weight(a, a.Ntd[d], td, q, &wd, &dwd);
+    if (tap) {
+      if (poly) {
+        const double x = td - (double)a.m + 0.5;
+        double v = cq[W], dv = 0.0;
+#pragma unroll
+        for (int i = W - 1; i >= 0; --i) {
+          dv = fma(x, dv, v);
+          v = fma(x, v, cq[i]);
+        }
+        wd = v;
+        dwd = a.Ntd[d] * dv;
+      } else {
+        tap_weight(a, a.Ntd[d], td, q, &wd, &dwd);
+      }
+    }
     w[d] = (R)wd;
     dw[d] = (R)dwd;
     base[d] = (long long)cw - a.m + 1;
@@ -194,8 +220,6 @@ __global__ __launch_bounds__(256) void k_grad_lanes(KArgs a, const typename cpx<
       gi[d] += __shfl_xor(gi[d], o, G);
     }
   if (q == 0) {
-    const int jj = a.perm[s];
-    const C y = ybar[jj];
 #pragma unroll
     for (int d = 0; d < DD; ++d)
       kbar[(size_t)d + (size_t)DD * jj] = (double)y.x * (double)gr[d] + (double)y.y * (double)gi[d];
