@@ -112,6 +112,8 @@ struct KArgs {
   int recBytes;
   int blocked3;       // 3-D scratch slots stored as 27 halo/interior blocks
   int tvar;           // 3-D tile geometry variant (compile-time tile edge)
+  int fix2;           // FP32 2-D spreading accumulates packed fixed point (one
+                      // 64-bit LDS atomic per tap) instead of two FP64 planes
   // spreading
   double wmaxD;        // bound of any product of D window values
   void* scratch;       // nitems padded tiles (dense, px fastest)
@@ -494,13 +496,46 @@ __device__ __forceinline__ void spread_tile_item(const KArgs& a, int item,
 
   for (int i = threadIdx.x; i < 2 * a.tileElems; i += NT) ((A*)tile)[i] = 0;
 
+  // FP32 2-D with fix2: packed 32+32-bit fixed point in the first tileElems
+  // 64-bit words, scale s = 2^(30 - e) from a strict bound of the item's
+  // sums, B = sum_j |f_j|_inf * wmaxD < 2^e (as the 3-D FP32 kernel); one
+  // ds_add_u64 per tap instead of two ds_add_f64
+  const bool fix = sizeof(R) == 4 && D == 2 && a.fix2;
+  R fscale = 1, inv = 1;
+  if (fix) {
+    R part = 0;
+    for (int j = it.y + (int)threadIdx.x; j < it.z; j += NT) {
+      const int jj = ((const int*)(a.rec + (size_t)j * NodeRec<R, D, W>::BYTES + NodeRec<R, D, W>::WBYTES))[1];
+      const C v = fhat[(long long)jj * a.fst];
+      part += fmax(fabs(v.x), fabs(v.y));
+    }
+    part = group_sum<64>(part);
+    R* red = (R*)lrec;
+    if (lane == 0) red[wave] = part;
+    __syncthreads();
+    double B = 0;
+    for (int w = 0; w < NW; ++w) B += (double)red[w];
+    B *= a.oscale * a.wmaxD * 1.0000001;
+    int e = 0;
+    if (B > 0) (void)frexp(B, &e);
+    // both powers stay normal floats; a non-finite bound (Inf or NaN data)
+    // spreads zeros and turns the slot into NaN at the flush
+    e = max(e, -96);
+    fscale = (R)ldexp(1.0, 30 - e);
+    inv = (R)ldexp(1.0, e - 30);
+    if (!(B <= 1.0e300)) {
+      fscale = 0;
+      inv = (R)NAN;
+    }
+  }
+
   constexpr int G = LanesPerNode<W>::value;
   constexpr int NPW = 64 / G;
   const int slot = lane / G, lx = lane & (G - 1);
   for (int b0 = it.y; b0 < it.z; b0 += NB) {
     const int nb = min(NB, it.z - b0);
     __syncthreads();
-    phase_a<R, D, W, NT, true>(a, fhat, b0, nb, lrec, fv, (R)a.oscale);
+    phase_a<R, D, W, NT, true>(a, fhat, b0, nb, lrec, fv, (R)a.oscale * fscale);
     __syncthreads();
     for (int n0 = wave * NPW; n0 < nb; n0 += NW * NPW) {
       const int n = n0 + slot;
@@ -512,10 +547,18 @@ __device__ __forceinline__ void spread_tile_item(const KArgs& a, int item,
         const R vr = f.x * w1, vi = f.y * w1;
         if constexpr (D == 2) {
           const int base = ((int)(pk & LM) + lx) + a.Sx * (int)(pk >> 16);
+          if (fix) {
 #pragma unroll
-          for (int ly = 0; ly < W; ++ly) {
-            const R w2 = w[W + ly];
-            SpreadCell<A>::add(tile, a.tileElems, base + a.Sx * ly, (A)(vr * w2), (A)(vi * w2));
+            for (int ly = 0; ly < W; ++ly) {
+              const R w2 = w[W + ly];
+              SpreadCell<float>::add(tile, a.tileElems, base + a.Sx * ly, (float)(vr * w2), (float)(vi * w2));
+            }
+          } else {
+#pragma unroll
+            for (int ly = 0; ly < W; ++ly) {
+              const R w2 = w[W + ly];
+              SpreadCell<A>::add(tile, a.tileElems, base + a.Sx * ly, (A)(vr * w2), (A)(vi * w2));
+            }
           }
         } else {
           SpreadCell<A>::add(tile, a.tileElems, (int)pk + lx, (A)vr, (A)vi);
@@ -534,8 +577,13 @@ __device__ __forceinline__ void spread_tile_item(const KArgs& a, int item,
       px = i - q * a.Pd[0];
       py = q;
     }
-    const auto v = SpreadCell<A>::get(tile, a.tileElems, px + a.Sx * py, 1.0);
-    out[i] = C{(R)v.x, (R)v.y};
+    if (fix) {
+      const float2 v = SpreadCell<float>::get(tile, a.tileElems, px + a.Sx * py, (float)inv);
+      out[i] = C{(R)v.x, (R)v.y};
+    } else {
+      const auto v = SpreadCell<A>::get(tile, a.tileElems, px + a.Sx * py, 1.0);
+      out[i] = C{(R)v.x, (R)v.y};
+    }
   }
 }
 

@@ -1,6 +1,7 @@
-"""Stress of the FP32 3-D spreading, which accumulates each work item's
-padded tile in packed 32+32-bit fixed point with a per-item scale
-2^(30 - e), 2^e > sum_j |fHat_j|_inf * max(w)^3.
+"""Stress of the FP32 3-D spreading (and the FP32 2-D spreading at sigma >=
+1.9), which accumulates each work item's padded tile in packed 32+32-bit
+fixed point with a per-item scale 2^(30 - e), 2^e > sum_j |fHat_j|_inf *
+max(w)^D.
 
 Its rounding error grows with the nodes per work item (up to 1024) and with
 the dynamic range of fHat, so the cases here are the hard ones:
@@ -124,3 +125,43 @@ def test_crowded_tile_wide_range_large_grid():
     # a second call on other data through the same plan
     fh2 = wide_range(rng, k.shape[1], 1e-2, 1e2)
     assert rel(p32.H * fh2, p64.H * fh2) < 1e-5
+
+
+# FP32 2-D spreading (sigma >= 1.9) accumulates in the same packed fixed point
+# with a per-item bound over at most 2048 nodes (twice the mean tile load).
+
+def test_2d_crowded_tiles_wide_range():
+    print("FP32 2-D fixed point: crowded tiles, fHat over 1e-2..1e4", flush=True)
+    rng = np.random.default_rng([91, 5])
+    N = (256, 256)
+    crowd = 0.02 * rng.random((2, 20000))  # the tiles at the origin, many full items
+    k = np.concatenate([crowd, rng.random((2, 40000)) - 0.5], axis=1)
+    fh = wide_range(rng, k.shape[1], 1e-2, 1e4)
+    fh[rng.choice(k.shape[1], 10, replace=False)] = 1e-30
+    p32, p64 = plans(k, N)
+    e = rel(p32.H * fh, p64.H * fh)
+    g32 = torch.empty(p32.Ntilde, dtype=torch.complex64, device="cuda")
+    g64 = torch.empty(p64.Ntilde, dtype=torch.complex128, device="cuda")
+    p32.convolve_transpose_(torch.as_tensor(fh), g32)
+    p64.convolve_transpose_(torch.as_tensor(fh), g64)
+    eg = rel(g32, g64)
+    print(f"adjoint FP32 vs FP64 {e:.3e}, convolve_transpose! {eg:.3e}", flush=True)
+    assert e < 1e-5 and eg < 1e-5
+
+
+def test_2d_tiny_and_nonfinite_data():
+    print("FP32 2-D fixed point: 1e-30 data, and Inf", flush=True)
+    rng = np.random.default_rng([91, 6])
+    N = (128, 96)
+    k = rng.random((2, 30000)) - 0.5
+    fh = (rng.standard_normal(k.shape[1]) + 1j * rng.standard_normal(k.shape[1])).astype(np.complex64)
+    p32, p64 = plans(k, N)
+    x1 = (p32.H * fh).cpu().numpy().astype(np.complex128)
+    tiny = (fh.astype(np.complex128) * 1e-30).astype(np.complex64)
+    xt = (p32.H * tiny).cpu().numpy().astype(np.complex128)
+    assert O.relerr(xt * 1e30, x1) < 1e-5
+    assert rel(p64.H * fh, x1) < 1e-5
+    bad = fh.copy()
+    bad[77] = np.inf
+    assert not np.all(np.isfinite((p32.H * bad).cpu().numpy()))
+    assert np.all(np.isfinite((p32.H * fh).cpu().numpy()))
