@@ -1,0 +1,36 @@
+// fft1.hpp -- the FFT stage of 1-D transforms on large grids (k_fft1.hip):
+// Nt = n1 Lm^2 points in three passes of line FFTs, the deconvolution fused
+// into the first forward pass and the last adjoint pass.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nfftm {
+
+struct F1Args {
+  int n1, P, M, Lm;  // n1 = P M (P = 1 or 5, M a power of two), n2 = n3 = Lm
+  int N, Na, Nt;     // image size, N / 2, grid size n1 Lm^2
+  long long ist;     // element stride of the image
+  int sh;            // W^e = thi[e >> sh] tlo[e mod 2^sh], W = exp(-+2 pi i / Nt)
+  const double* fac;  // deconvolution factors (N)
+  const void* tw1;    // (cos, sin)(2 pi j / n1), j < n1
+  const void* twl;    // (cos, sin)(2 pi j / Lm), j < Lm
+  const void* thi;    // (cos, sin)(2 pi (j << sh) / Nt), j <= (Nt - 1) >> sh
+  const void* tlo;    // (cos, sin)(2 pi j / Nt), j < 2^sh
+  double oscale;
+};
+
+// n1 and Lm with Nt = n1 Lm^2 for which the passes are built (Lm in {256,
+// 128, 64}; n1 = 2^k in [64, 1024] or 5 2^k in [80, 1280], FP64 at most
+// 640 for the latter; n1 a multiple of the lines per workgroup of the
+// strided passes), else false.
+bool fft1_factor(int64_t Nt, int64_t N, bool dbl, int* n1, int* Lm);
+// Forward: part 0 = the first pass (f deconvolved into the grid), part 1 =
+// the other two passes, in place.
+hipError_t launch_fft1_fwd(bool dbl, const F1Args& a, hipStream_t s, const void* f, void* grid, int part);
+// Adjoint: part 0 = the first two passes, in place, part 1 = the last pass
+// into f with the transposed deconvolution.
+hipError_t launch_fft1_adj(bool dbl, const F1Args& a, hipStream_t s, void* grid, void* f, int part);
+
+}  // namespace nfftm

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/nfftm.h"
+#include "fft1.hpp"
 #include "launch.hpp"
 
 using namespace nfftm;
@@ -476,6 +477,11 @@ struct nfftm_plan_s {
   bool pfft = false;
   DevBuf<char> d_tw;
   size_t tw_off[3] = {0, 0, 0};
+  // 1-D grids of n1 Lm^2 points: the three-pass FFT of k_fft1.hip, its
+  // tables (n1-point, Lm-point, and the two of exp(2 pi i e / Nt)) in d_tw
+  bool f1 = false;
+  int f1n1 = 0, f1Lm = 0, f1sh = 0;
+  size_t f1off[4] = {0, 0, 0, 0};
   double timing[7] = {0};
   // kernel-level profiling (event pairs recorded without host sync)
   bool profiling = false;
@@ -626,6 +632,27 @@ struct nfftm_plan_s {
     }
     a.oscale = 1.0;
     a.ist = io_stride;
+    return a;
+  }
+
+  F1Args f1args() const {
+    F1Args a;
+    std::memset(&a, 0, sizeof a);
+    a.n1 = f1n1;
+    a.P = f1n1 % 5 == 0 ? 5 : 1;
+    a.M = f1n1 / a.P;
+    a.Lm = f1Lm;
+    a.N = (int)N[0];
+    a.Na = (int)(N[0] / 2);
+    a.Nt = (int)Nt[0];
+    a.ist = io_stride;
+    a.sh = f1sh;
+    a.fac = d_deconv.p;
+    a.tw1 = d_tw.p + f1off[0];
+    a.twl = d_tw.p + f1off[1];
+    a.thi = d_tw.p + f1off[2];
+    a.tlo = d_tw.p + f1off[3];
+    a.oscale = 1.0;
     return a;
   }
 
@@ -1103,6 +1130,17 @@ int forward_to_grid(nfftm_plan_s& p, const void* f, Mark&& mark) {
     HIPCHK(launch_pruned_fft_fwd(p.dbl(), a, p.stream, f, p.d_grid.p, 1));
     return NFFTM_OK;
   }
+  if (p.f1) {
+    const F1Args a = p.f1args();
+    {
+      ProfScope ps(p, 0);
+      HIPCHK(launch_fft1_fwd(p.dbl(), a, p.stream, f, p.d_grid.p, 0));
+    }
+    if (int rc = mark()) return rc;
+    ProfScope ps(p, 1);
+    HIPCHK(launch_fft1_fwd(p.dbl(), a, p.stream, f, p.d_grid.p, 1));
+    return NFFTM_OK;
+  }
   if (int rc = run_deconv(p, true, f, p.d_grid.p)) return rc;
   if (int rc = mark()) return rc;
   ProfScope ps(p, 1);
@@ -1127,6 +1165,17 @@ int grid_to_image(nfftm_plan_s& p, void* f, Mark&& mark, bool fused = false) {
     if (int rc = mark()) return rc;
     ProfScope ps(p, 6);
     HIPCHK(launch_pruned_fft_adj(p.dbl(), a, p.stream, p.d_grid.p, f, 1));
+    return NFFTM_OK;
+  }
+  if (p.f1) {
+    const F1Args a = p.f1args();
+    {
+      ProfScope ps(p, 5);
+      HIPCHK(launch_fft1_adj(p.dbl(), a, p.stream, p.d_grid.p, f, 0));
+    }
+    if (int rc = mark()) return rc;
+    ProfScope ps(p, 6);
+    HIPCHK(launch_fft1_adj(p.dbl(), a, p.stream, p.d_grid.p, f, 1));
     return NFFTM_OK;
   }
   {
@@ -1213,6 +1262,41 @@ int create_impl(nfftm_plan_s& p, const double* k) {
         }
       }
     }
+    HIPCHK(p.d_tw.alloc(bytes));
+    HIPCHK(hipMemcpy(p.d_tw.p, h.data(), bytes, hipMemcpyHostToDevice));
+  }
+  p.f1 = !p.pfft && p.D == 1 && !(p.flags & NFFTM_FLAG_ROCFFT) &&
+         fft1_factor(p.Nt[0], p.N[0], p.dbl(), &p.f1n1, &p.f1Lm);
+  if (p.f1) {
+    // (cos, sin)(2 pi j / n1), (2 pi j / Lm), (2 pi (j << sh) / Nt) and
+    // (2 pi j / Nt), 2^sh >= sqrt(Nt): FP64 on the host, stored in the data
+    // precision
+    const i64 Nt = p.Nt[0];
+    int sh = 0;
+    while ((1LL << (2 * sh)) < Nt) ++sh;
+    p.f1sh = sh;
+    const i64 len[4] = {p.f1n1, p.f1Lm, ((Nt - 1) >> sh) + 1, 1LL << sh};
+    const i64 per[4] = {p.f1n1, p.f1Lm, Nt, Nt};
+    const size_t es = p.dbl() ? 16 : 8;
+    size_t bytes = 0;
+    for (int i = 0; i < 4; ++i) {
+      p.f1off[i] = bytes;
+      bytes += (size_t)len[i] * es;
+    }
+    std::vector<char> h(bytes);
+    for (int i = 0; i < 4; ++i)
+      for (i64 j = 0; j < len[i]; ++j) {
+        const i64 e = (i == 2 ? (j << sh) : j) % per[i];
+        const double th = 2.0 * M_PI * (double)e / (double)per[i];
+        char* q = h.data() + p.f1off[i] + (size_t)j * es;
+        if (p.dbl()) {
+          ((double*)q)[0] = std::cos(th);
+          ((double*)q)[1] = std::sin(th);
+        } else {
+          ((float*)q)[0] = (float)std::cos(th);
+          ((float*)q)[1] = (float)std::sin(th);
+        }
+      }
     HIPCHK(p.d_tw.alloc(bytes));
     HIPCHK(hipMemcpy(p.d_tw.p, h.data(), bytes, hipMemcpyHostToDevice));
   }

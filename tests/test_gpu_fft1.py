@@ -1,0 +1,131 @@
+"""Three-pass FFT stage of 1-D plans on large grids (k_fft1.hip) against the
+whole-grid rocFFT stage and the CPU oracle.
+
+1-D grids of Nt = n1 Lm^2 points (Lm = 256, 128 or 64; n1 = 2^k or 5 2^k)
+run their FFT stage as three passes of line FFTs with the deconvolution
+fused into the first forward and the last adjoint pass (the default);
+``rocfft=True`` selects rocFFT over the whole grid with separate
+deconvolution passes.  Both must give the same transforms: FP64 rel-l2
+<= 1e-13 between the two paths and <= 1e-12 against the oracle; FP32
+<= 2e-6 between the paths.  The cases cover n1 = 5 M for M = 16 .. 256
+(radix-5 combine), n1 = 2^k, each Lm, odd and even N (the fftshift split
+Na = N / 2 of src/deconvolution.jl:22-45), and the contention-stress grid
+of N = 2^24 at sigma = 1.25.
+"""
+import numpy as np
+import pytest
+import torch
+
+import nfftm
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu().numpy()
+    if isinstance(b, torch.Tensor):
+        b = b.detach().cpu().numpy()
+    return O.relerr(a, b)
+
+
+def crandn(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# (N, Nt): n1 x Lm^2 in the comment
+CASES = [
+    (1 << 20, 1310720),        # 320 x 64^2 (5 x 64)
+    (1 << 19, 1 << 20),        # 64 x 128^2
+    (3 * (1 << 18) + 1, 5242880),  # 80 x 256^2 (5 x 16), odd N
+    (1 << 21, 2621440),        # 160 x 128^2 (5 x 32)
+    (1 << 22, 5 * (1 << 21)),  # 160 x 256^2 (5 x 32)
+    (1 << 23, 1 << 24),        # 256 x 256^2
+    (1 << 24, 20971520),       # 320 x 256^2 (5 x 64)
+    (3000000, 1 << 25),        # 512 x 256^2
+]
+F32_ONLY = [
+    (1 << 24, 1 << 26),        # 1024 x 256^2
+    (1 << 25, 5 * (1 << 24)),  # 1280 x 256^2 (5 x 256)
+]
+
+
+def make(k, N, Nt, dtype, rocfft, precompute=nfftm.LINEAR):
+    p = nfftm.plan_nfft(k, (N,), m=4, sigma=Nt / N, precompute=precompute, dtype=dtype, rocfft=rocfft)
+    assert p.Ntilde == (Nt,), (p.Ntilde, Nt)
+    return p
+
+
+def run_case(N, Nt, dtype, M=20000, oracle=False):
+    print(f"fft1 N={N} Nt={Nt} dtype={dtype}", flush=True)
+    rng = np.random.default_rng([N % 1000003, Nt % 1000003, dtype == torch.float64])
+    k = rng.random((1, M)) - 0.5
+    p = make(k, N, Nt, dtype, False)
+    q = make(k, N, Nt, dtype, True)
+    cd = torch.complex128 if dtype == torch.float64 else torch.complex64
+    f = torch.as_tensor(crandn(rng, (N,)), dtype=cd, device="cuda")
+    fh = torch.as_tensor(crandn(rng, M), dtype=cd, device="cuda")
+    tol = 1e-13 if dtype == torch.float64 else 2e-6
+    y, yq = p * f, q * f
+    x, xq = p.H * fh, q.H * fh
+    torch.cuda.synchronize()
+    assert rel(y, yq) < tol
+    assert rel(x, xq) < tol
+    if oracle:
+        orc = O.OraclePlan(k, (N,), m=4, sigma=Nt / N, precompute=O.LINEAR)
+        assert orc.Nt == (Nt,)
+        assert rel(y, orc.forward(f.cpu().numpy())) < 1e-12
+        assert rel(x, orc.adjoint(fh.cpu().numpy())) < 1e-12
+
+
+@pytest.mark.parametrize("N,Nt", CASES, ids=[f"N{c[0]}_Nt{c[1]}" for c in CASES])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_fft1_matches_rocfft(N, Nt, dtype):
+    run_case(N, Nt, dtype, oracle=dtype == torch.float64 and Nt <= (1 << 21))
+
+
+@pytest.mark.parametrize("N,Nt", F32_ONLY, ids=[f"N{c[0]}_Nt{c[1]}" for c in F32_ONLY])
+def test_fft1_matches_rocfft_large_f32(N, Nt):
+    run_case(N, Nt, torch.float32, M=5000)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32], ids=["f64", "f32"])
+def test_fft1_grad_timing_and_repeat(dtype):
+    """Node gradient (forward FFT stage), TimingStats (the fused first pass
+    counts as deconvolution, the other passes as the FFT) and repeated
+    transforms with other GPU work in between, TENSOR precompute."""
+    rng = np.random.default_rng(91)
+    N, Nt, M = 1 << 20, 1310720, 30000
+    k = rng.random((1, M)) - 0.5
+    p = make(k, N, Nt, dtype, False, nfftm.TENSOR)
+    q = make(k, N, Nt, dtype, True, nfftm.TENSOR)
+    cd = torch.complex128 if dtype == torch.float64 else torch.complex64
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    t = nfftm.TimingStats()
+    for it in range(3):
+        f = torch.as_tensor(crandn(rng, (N,)), dtype=cd, device="cuda")
+        fh = torch.as_tensor(crandn(rng, M), dtype=cd, device="cuda")
+        g1 = p.grad_nodes(f, fh)
+        g2 = q.grad_nodes(f, fh)
+        x = p.adjoint(fh, timing=t)
+        _ = torch.randn(1 << 18, device="cuda").sum()
+        y = p.forward(f, timing=t)
+        torch.cuda.synchronize()
+        assert rel(g1, g2) < tol, it
+        assert rel(x, q.H * fh) < tol, it
+        assert rel(y, q * f) < tol, it
+    for key in ("conv", "fft", "deconv", "conv_adjoint", "fft_adjoint", "deconv_adjoint"):
+        assert getattr(t, key) > 0, (key, t)
+
+
+def test_fft1_not_used_off_the_factorisations():
+    """Grids outside the factorisations keep the rocFFT stage and still match
+    the oracle (Nt = 3 x 2^18 has no n1 x Lm^2 split with a supported n1)."""
+    rng = np.random.default_rng(3)
+    N, Nt, M = 1 << 18, 786432, 5000
+    k = rng.random((1, M)) - 0.5
+    p = make(k, N, Nt, torch.float64, False)
+    f = torch.as_tensor(crandn(rng, (N,)), dtype=torch.complex128, device="cuda")
+    orc = O.OraclePlan(k, (N,), m=4, sigma=Nt / N, precompute=O.LINEAR)
+    assert rel(p * f, orc.forward(f.cpu().numpy())) < 1e-12
