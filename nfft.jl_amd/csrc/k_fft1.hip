@@ -23,6 +23,12 @@
 // positions (only the image positions, the transposed deconvolution fused).
 // The forward fills the whole grid and the adjoint reads all of it.
 //
+// Pass 1 reads the deconvolution factors from a copy laid out per workgroup
+// (fft1_lines): slab (j2, j3 / B), then j1, then j3 mod B, so that each
+// workgroup reads its factors as one contiguous block (read at the image
+// positions of the strided side, as the data are, they cost the adjoint's
+// last pass 70 of its 150 us on a 2^22 x 5 grid).
+//
 // The strided passes take LPB lines consecutive in a, so that each load and
 // store of a wavefront is a run of LPB consecutive elements; pass 1 takes B
 // lines consecutive in j3 for the same reason on its strided side.  Pass 1
@@ -31,9 +37,12 @@
 //   X[k' + q M] = sum_r W_5^(r q) W_n1^(r k') S_r[k'].
 // Values equal the full FFT of the zero-padded grid (FFTW's forward /
 // unnormalised backward transform, src/implementation.jl:87-88) up to
-// rounding.  Twiddles of W come from two tables of (cos, sin) pairs, FP64 on
-// the host and stored in the data precision: W^e = thi[e >> sh] tlo[e mod
-// 2^sh].
+// rounding.  Twiddles of W come from two FP64 tables of (cos, sin) pairs:
+// W^e = thi[e >> sh] tlo[e mod 2^sh].  A thread of a strided pass looks up
+// W^(al j0 + be) and W^(al L / 16) once and steps through its 16 points by
+// FP64 products (table gathers per point, scattered over the tables, cost
+// the passes 25-55 us each on a 2^22 x 5 grid); pass 1 tabulates
+// W^(n3 j2 k1), k1 < n1, per workgroup in LDS.
 #include "fft1.hpp"
 #include "fft_line.hpp"
 
@@ -64,16 +73,18 @@ __device__ __forceinline__ void dft5(C* v) {
   v[3] = csub(t2, m2);
 }
 
-// (cos, sin)(2 pi e / Nt), 0 <= e < Nt
-template <typename C>
-__device__ __forceinline__ C big_tw(const F1Args& a, int e) {
-  const C h = ((const C*)a.thi)[e >> a.sh];
-  const C l = ((const C*)a.tlo)[e & ((1 << a.sh) - 1)];
-  return C{h.x * l.x - h.y * l.y, h.y * l.x + h.x * l.y};
+using D2 = cpx<double>::T;
+
+__device__ __forceinline__ D2 cmul(D2 x, D2 y) { return D2{x.x * y.x - x.y * y.y, x.y * y.x + x.x * y.y}; }
+
+// (cos, sin)(2 pi e / Nt), 0 <= e < Nt, FP64
+__device__ __forceinline__ D2 big_tw(const F1Args& a, int e) {
+  return cmul(((const D2*)a.thi)[e >> a.sh], ((const D2*)a.tlo)[e & ((1 << a.sh) - 1)]);
 }
 
-// Pass 1 geometry: B lines of n1 = P M points per workgroup, P B sub-lines
-// of M points (LDS stride M + 1), M / 16 threads per sub-line.
+// Pass 1 geometry (B as fft1_lines): B lines of n1 = P M points per workgroup, P B sub-lines
+// of M points (LDS stride M + 1), M / 16 threads per sub-line; LDS also
+// holds the M-point and n1-point line twiddles and the forward's W^(n3 j2 k1).
 template <typename R, int P, int M>
 struct P1Geo {
   static constexpr int L = P * M;
@@ -81,7 +92,7 @@ struct P1Geo {
   static constexpr int TLm = M / 16;
   static constexpr int NT = B * P * TLm;
   static constexpr int SS = M + 1;
-  static constexpr int LDS_C = B * P * SS + M + (P > 1 ? L : 0);  // complex elements
+  static constexpr int LDS_C = B * P * SS + M + (P > 1 ? L : 0) + L;  // complex elements
   static_assert(LDS_C * sizeof(typename cpx<R>::T) <= 65536, "pass-1 LDS");
 };
 
@@ -104,24 +115,38 @@ __global__ __launch_bounds__((P1Geo<R, P, M>::NT)) void k_fft1_first(F1Args a, c
   C* buf = sm;
   C* twm = sm + G::B * P * G::SS;
   C* twn = twm + M;
-  p1_tables<R, P, M>(a, twm, twn);
+  C* tb = twn + (P > 1 ? G::L : 0);
   const int Lm = a.Lm, n23 = a.Lm * a.Lm;
   const int j2 = blockIdx.y, j3_0 = blockIdx.x * G::B;
   const int t = threadIdx.x;
   // loads: line fastest (consecutive grid positions), then sub-line, then j0
   const int l = t % G::B, r = (t / G::B) % P, j0 = t / (G::B * P);
   const int g0 = j2 * Lm + j3_0 + l;
+  const double* fb = a.fac1 + (size_t)(j2 * (Lm / G::B) + blockIdx.x) * a.n1 * G::B + l;
+  // all 32 loads issued before any is used (unconditional: positions
+  // outside the image load element 0 and are zeroed after)
   C x[16];
+  double fr[16];
+  unsigned in = 0;
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
-    const int g = (P * (j0 + u * G::TLm) + r) * n23 + g0;
-    const int ix = image_idx(g, a.N, a.Nt, a.Na);
-    x[u] = C{0, 0};
-    if (ix >= 0) {
-      const C v = f[(long long)ix * a.ist];
-      const double s = a.fac[ix] * a.oscale;
-      x[u] = C{(R)((double)v.x * s), (R)((double)v.y * s)};
-    }
+    const int j1 = P * (j0 + u * G::TLm) + r;
+    const int ix = image_idx(j1 * n23 + g0, a.N, a.Nt, a.Na);
+    in |= ix >= 0 ? 1u << u : 0u;
+    x[u] = f[(long long)(ix >= 0 ? ix : 0) * a.ist];
+    fr[u] = fb[ix >= 0 ? j1 * G::B : 0];
+  }
+  // the tables after the data loads (a wait for a table load also waits for
+  // the data loads issued before it)
+  p1_tables<R, P, M>(a, twm, twn);
+  for (int i = threadIdx.x; i < G::L; i += G::NT) {
+    const D2 w = big_tw(a, Lm * j2 * i);
+    tb[i] = C{(R)w.x, (R)w.y};
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const double s = fr[u] * a.oscale;
+    x[u] = (in >> u) & 1 ? C{(R)((double)x[u].x * s), (R)((double)x[u].y * s)} : C{0, 0};
   }
   line_fft<R, M, false>(x, buf + (l * P + r) * G::SS, j0, buf + (t / G::TLm) * G::SS, t % G::TLm, twm);
   __syncthreads();
@@ -147,7 +172,7 @@ __global__ __launch_bounds__((P1Geo<R, P, M>::NT)) void k_fft1_first(F1Args a, c
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int k1 = kp + q * M;
-      dst[k1] = tmul<false>(v[q], big_tw<C>(a, Lm * j2 * k1));
+      dst[k1] = tmul<false>(v[q], tb[k1]);
     }
   }
 }
@@ -163,7 +188,6 @@ __global__ __launch_bounds__((P1Geo<R, P, M>::NT)) void k_fft1_last(F1Args a, co
   C* buf = sm;
   C* twm = sm + G::B * P * G::SS;
   C* twn = twm + M;
-  p1_tables<R, P, M>(a, twm, twn);
   const int Lm = a.Lm, n23 = a.Lm * a.Lm;
   const int j2 = blockIdx.y, j3_0 = blockIdx.x * G::B;
   const int t = threadIdx.x;
@@ -173,6 +197,14 @@ __global__ __launch_bounds__((P1Geo<R, P, M>::NT)) void k_fft1_last(F1Args a, co
   C x[16];
 #pragma unroll
   for (int u = 0; u < 16; ++u) x[u] = src[P * (j0 + u * G::TLm) + r];
+  // factors of the next butterfly's outputs loaded one butterfly ahead (the
+  // first with the data), so that their latency overlaps the work before
+  // they are used (zero at the positions that are not written)
+  const double* fb = a.fac1 + (size_t)(j2 * (Lm / G::B) + blockIdx.x) * a.n1 * G::B;
+  double fcur[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) fcur[q] = fb[((t / G::B) + q * M) * G::B + t % G::B];
+  p1_tables<R, P, M>(a, twm, twn);
   line_fft<R, M, true>(x, buf + (l * P + r) * G::SS, j0, buf + (t / G::TLm) * G::SS, t % G::TLm, twm);
   __syncthreads();
   {
@@ -185,6 +217,10 @@ __global__ __launch_bounds__((P1Geo<R, P, M>::NT)) void k_fft1_last(F1Args a, co
   // grid positions j1 n2 n3 + j2 n3 + j3)
   for (int idx = t; idx < G::B * M; idx += G::NT) {
     const int ll = idx % G::B, kp = idx / G::B;
+    double fnext[P];
+    const int nx = idx + G::NT < G::B * M ? idx + G::NT : idx;
+#pragma unroll
+    for (int q = 0; q < P; ++q) fnext[q] = fb[((nx / G::B) + q * M) * G::B + nx % G::B];
     C v[P];
 #pragma unroll
     for (int q = 0; q < P; ++q) v[q] = buf[(ll * P + q) * G::SS + kp];
@@ -198,9 +234,10 @@ __global__ __launch_bounds__((P1Geo<R, P, M>::NT)) void k_fft1_last(F1Args a, co
       const int g = (kp + q * M) * n23 + j2 * Lm + j3_0 + ll;
       const int ix = image_idx(g, a.N, a.Nt, a.Na);
       if (ix >= 0) {
-        const double s = a.fac[ix] * a.oscale;
+        const double s = fcur[q] * a.oscale;
         f[(long long)ix * a.ist] = C{(R)((double)v[q].x * s), (R)((double)v[q].y * s)};
       }
+      fcur[q] = fnext[q];
     }
   }
 }
@@ -221,20 +258,29 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft1_mid(F1Args a, type
   __shared__ __align__(16) R sm[2 * (G::LPB * G::LS + L)];
   C* buf = (C*)sm;
   C* tw = buf + G::LPB * G::LS;
-  load_tw<G::NT>(tw, a.twl, L);
   const int a0 = blockIdx.x * G::LPB, y = blockIdx.y;
   const int xl = threadIdx.x % G::LPB, j0 = threadIdx.x / G::LPB;
   C* base = grid + (size_t)y * ystr + a0 + xl;
   C v[16];
 #pragma unroll
   for (int u = 0; u < 16; ++u) v[u] = base[(size_t)(j0 + u * G::TL) * st];
-  line_fft<R, L, INV>(v, buf + xl * G::LS, j0, buf + (threadIdx.x / G::TL) * G::LS, threadIdx.x % G::TL, tw);
+  // twiddle loads with the data, the line table after them
+  D2 w{1, 0}, st16{1, 0};
   if constexpr (TW != 0) {
     const int aa = a0 + xl;
     const int al = TW == 1 ? a.n1 * y : (TW == 2 ? aa + a.n1 * y : a.Lm * aa);
     const int be = TW == 1 ? y * aa : 0;
+    w = big_tw(a, al * j0 + be);
+    st16 = big_tw(a, al * G::TL);
+  }
+  load_tw<G::NT>(tw, a.twl, L);
+  line_fft<R, L, INV>(v, buf + xl * G::LS, j0, buf + (threadIdx.x / G::TL) * G::LS, threadIdx.x % G::TL, tw);
+  if constexpr (TW != 0) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = tmul<INV>(v[u], big_tw<C>(a, al * (j0 + u * G::TL) + be));
+    for (int u = 0; u < 16; ++u) {
+      v[u] = tmul<INV>(v[u], C{(R)w.x, (R)w.y});
+      w = cmul(w, st16);
+    }
   }
 #pragma unroll
   for (int u = 0; u < 16; ++u) base[(size_t)(j0 + u * G::TL) * st] = v[u];
@@ -244,6 +290,7 @@ template <typename R, int P, int M>
 hipError_t go_p1(bool fwd, const F1Args& a, hipStream_t s, const void* in, void* out) {
   using C = typename cpx<R>::T;
   using G = P1Geo<R, P, M>;
+  if (a.B != G::B) return hipErrorInvalidValue;
   const dim3 g((unsigned)(a.Lm / G::B), (unsigned)a.Lm), b(G::NT);
   if (fwd) hipLaunchKernelGGL((k_fft1_first<R, P, M>), g, b, 0, s, a, (const C*)in, (C*)out);
   else hipLaunchKernelGGL((k_fft1_last<R, P, M>), g, b, 0, s, a, (const C*)in, (C*)out);
@@ -270,7 +317,9 @@ hipError_t pass1(bool fwd, const F1Args& a, hipStream_t s, const void* in, void*
     case 128: return go_p1<R, 1, 128>(fwd, a, s, in, out);
     case 256: return go_p1<R, 1, 256>(fwd, a, s, in, out);
     case 512: return go_p1<R, 1, 512>(fwd, a, s, in, out);
-    case 1024: return go_p1<R, 1, 1024>(fwd, a, s, in, out);
+    case 1024:
+      if constexpr (sizeof(R) == 4) return go_p1<R, 1, 1024>(fwd, a, s, in, out);
+      return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }
@@ -325,6 +374,7 @@ bool fft1_factor(int64_t Nt, int64_t N, bool dbl, int* n1o, int* Lmo) {
     const int64_t M = n1 / P;
     if (M & (M - 1)) continue;
     if (P == 1 && (M < 64 || M > 1024)) continue;
+    if (P == 1 && dbl && M > 512) continue;
     if (P == 5 && (M < 16 || M > (dbl ? 128 : 256))) continue;
     if (n1 % ((dbl ? 2048 : 4096) / Lm)) continue;
     *n1o = (int)n1;

@@ -482,6 +482,7 @@ struct nfftm_plan_s {
   bool f1 = false;
   int f1n1 = 0, f1Lm = 0, f1sh = 0;
   size_t f1off[4] = {0, 0, 0, 0};
+  DevBuf<double> d_fac1;
   double timing[7] = {0};
   // kernel-level profiling (event pairs recorded without host sync)
   bool profiling = false;
@@ -647,7 +648,8 @@ struct nfftm_plan_s {
     a.Nt = (int)Nt[0];
     a.ist = io_stride;
     a.sh = f1sh;
-    a.fac = d_deconv.p;
+    a.B = fft1_lines(f1n1, dbl());
+    a.fac1 = d_fac1.p;
     a.tw1 = d_tw.p + f1off[0];
     a.twl = d_tw.p + f1off[1];
     a.thi = d_tw.p + f1off[2];
@@ -1268,28 +1270,27 @@ int create_impl(nfftm_plan_s& p, const double* k) {
   p.f1 = !p.pfft && p.D == 1 && !(p.flags & NFFTM_FLAG_ROCFFT) &&
          fft1_factor(p.Nt[0], p.N[0], p.dbl(), &p.f1n1, &p.f1Lm);
   if (p.f1) {
-    // (cos, sin)(2 pi j / n1), (2 pi j / Lm), (2 pi (j << sh) / Nt) and
-    // (2 pi j / Nt), 2^sh >= sqrt(Nt): FP64 on the host, stored in the data
-    // precision
+    // (cos, sin)(2 pi j / n1) and (2 pi j / Lm) in the data precision,
+    // (2 pi (j << sh) / Nt) and (2 pi j / Nt), 2^sh >= sqrt(Nt), in FP64
     const i64 Nt = p.Nt[0];
     int sh = 0;
     while ((1LL << (2 * sh)) < Nt) ++sh;
     p.f1sh = sh;
     const i64 len[4] = {p.f1n1, p.f1Lm, ((Nt - 1) >> sh) + 1, 1LL << sh};
     const i64 per[4] = {p.f1n1, p.f1Lm, Nt, Nt};
-    const size_t es = p.dbl() ? 16 : 8;
     size_t bytes = 0;
     for (int i = 0; i < 4; ++i) {
       p.f1off[i] = bytes;
-      bytes += (size_t)len[i] * es;
+      bytes += (size_t)len[i] * (p.dbl() || i >= 2 ? 16 : 8);
     }
     std::vector<char> h(bytes);
     for (int i = 0; i < 4; ++i)
       for (i64 j = 0; j < len[i]; ++j) {
         const i64 e = (i == 2 ? (j << sh) : j) % per[i];
         const double th = 2.0 * M_PI * (double)e / (double)per[i];
-        char* q = h.data() + p.f1off[i] + (size_t)j * es;
-        if (p.dbl()) {
+        const bool d64 = p.dbl() || i >= 2;
+        char* q = h.data() + p.f1off[i] + (size_t)j * (d64 ? 16 : 8);
+        if (d64) {
           ((double*)q)[0] = std::cos(th);
           ((double*)q)[1] = std::sin(th);
         } else {
@@ -1299,6 +1300,17 @@ int create_impl(nfftm_plan_s& p, const double* k) {
       }
     HIPCHK(p.d_tw.alloc(bytes));
     HIPCHK(hipMemcpy(p.d_tw.p, h.data(), bytes, hipMemcpyHostToDevice));
+    // deconvolution factors in the per-workgroup order of pass 1
+    const int B = fft1_lines(p.f1n1, p.dbl()), Lm = p.f1Lm;
+    const i64 N0 = p.N[0], Na = N0 / 2;
+    std::vector<double> f1((size_t)Nt, 0.0);
+    for (i64 i = 0; i < N0; ++i) {
+      const i64 g = i < Na ? i - Na + Nt : i - Na;
+      const i64 j1 = g / ((i64)Lm * Lm), j2 = g / Lm % Lm, j3 = g % Lm;
+      f1[(size_t)(((j2 * (Lm / B) + j3 / B) * p.f1n1 + j1) * B + j3 % B)] = p.deconv[0][(size_t)i];
+    }
+    HIPCHK(p.d_fac1.alloc(f1.size()));
+    HIPCHK(hipMemcpy(p.d_fac1.p, f1.data(), f1.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   for (auto& e : p.ev) HIPCHK(hipEventCreate(&e));
   return NFFTM_OK;
