@@ -77,7 +77,10 @@ enum {
                               dimension, FP64: 16..1024, D <= 3): pruned line
                               FFTs that skip the zero padding (forward) and
                               the discarded positions (adjoint), with the
-                              deconvolution fused.  Results agree to rounding;
+                              deconvolution fused; on 1-D grids of n1 Lm^2
+                              points (Lm = 64, 128, 256; n1 = 2^k or 5 2^k):
+                              three passes of line FFTs, deconvolution
+                              fused.  Results agree to rounding;
                               the grid (nfftm_grid) then holds the full FFT
                               after a forward transform and a partial one
                               after an adjoint. */
