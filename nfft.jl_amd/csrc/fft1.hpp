@@ -8,6 +8,8 @@
 
 namespace nfftm {
 
+struct KArgs;
+
 struct F1Args {
   int n1, P, M, Lm;  // n1 = P M (P = 1 or 5, M a power of two), n2 = n3 = Lm
   int N, Na, Nt;     // image size, N / 2, grid size n1 Lm^2
@@ -35,7 +37,11 @@ inline int fft1_lines(int n1, bool dbl) { return (dbl ? 8 : 16) / (n1 >= 1024 ? 
 // the other two passes, in place.
 hipError_t launch_fft1_fwd(bool dbl, const F1Args& a, hipStream_t s, const void* f, void* grid, int part);
 // Adjoint: part 0 = the first two passes, in place, part 1 = the last pass
-// into f with the transposed deconvolution.
+// into f with the transposed deconvolution, part 2 = the second pass alone
+// (after launch_fft1_adj_fold).
 hipError_t launch_fft1_adj(bool dbl, const F1Args& a, hipStream_t s, void* grid, void* f, int part);
+// Adjoint first pass reading the 1-D spreading slots of k (the fold of
+// k_fold fused): the grid needs no fold launch before it.
+hipError_t launch_fft1_adj_fold(bool dbl, const F1Args& a, const KArgs& k, hipStream_t s, void* grid);
 
 }  // namespace nfftm

@@ -43,6 +43,8 @@
 // FP64 products (table gathers per point, scattered over the tables, cost
 // the passes 25-55 us each on a 2^22 x 5 grid); pass 1 tabulates
 // W^(n3 j2 k1), k1 < n1, per workgroup in LDS.
+#include <cstring>
+
 #include "fft1.hpp"
 #include "fft_line.hpp"
 
@@ -250,8 +252,8 @@ __global__ __launch_bounds__((P1Geo<R, P, M>::NT)) void k_fft1_last(F1Args a, co
 //   TW 2: adjoint pass 3, y = k2: al = a + n1 y, be = 0;
 //   TW 3: adjoint pass 2, y = j3: al = n3 a, be = 0.
 // Every exponent is below Nt.
-template <typename R, int L, bool INV, int TW>
-__global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft1_mid(F1Args a, typename cpx<R>::T* __restrict__ grid,
+template <typename R, int L, bool INV, int TW, bool FOLD>
+__global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft1_mid(F1Args a, KArgs k, typename cpx<R>::T* __restrict__ grid,
                                                                    int st, int ystr) {
   using C = typename cpx<R>::T;
   using G = LineGeo<R, L>;
@@ -262,8 +264,44 @@ __global__ __launch_bounds__((LineGeo<R, L>::NT)) void k_fft1_mid(F1Args a, type
   const int xl = threadIdx.x % G::LPB, j0 = threadIdx.x / G::LPB;
   C* base = grid + (size_t)y * ystr + a0 + xl;
   C v[16];
+  if constexpr (FOLD) {
+    // the fold of the 1-D spreading slots (k_fold) at the line's points.
+    // Tile t's slot covers grid points [t T - m + 1, t T + T + m) (mod Nt):
+    // a point g = t T + gm with m <= gm < T - m + 1, away from the ends
+    // (covered by the last tile's slot through the wrap below
+    // ntiles T - Nt + m, by tile 0's at and above Nt - m + 1), lies in its
+    // own tile's slot only; in a tile of at most one work item its value is
+    // that item's slot value.  Every other point takes every covering slot
+    // (fold_sum).
+    const C* scr = (const C*)k.scratch;
+    const int T = k.T[0], m1 = k.m - 1, Ntg = k.Nt[0], glo = k.ntiles[0] * T - Ntg + k.m;
+    const int g0 = y * ystr + a0 + xl;
+    int s0[16], s1[16];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) v[u] = base[(size_t)(j0 + u * G::TL) * st];
+    for (int u = 0; u < 16; ++u) {
+      const int t = (g0 + (j0 + u * G::TL) * st) / T;
+      s0[u] = k.tstart[t];
+      s1[u] = k.tstart[t + 1];
+    }
+    unsigned slow = 0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int g = g0 + (j0 + u * G::TL) * st;
+      const int gm = g - (g / T) * T;
+      const bool fast = gm >= k.m && gm < T - m1 && g >= glo && g < Ntg - m1 && s1[u] - s0[u] <= 1;
+      slow |= fast ? 0u : 1u << u;
+      v[u] = fast && s1[u] > s0[u] ? scr[(size_t)s0[u] * k.padCells + gm + m1] : C{0, 0};
+    }
+    if (slow) {
+      const int z[1] = {0};
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if ((slow >> u) & 1u) v[u] = fold_sum<R, 1>(k, scr, g0 + (j0 + u * G::TL) * st, 1, z, z, 1, z, z);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = base[(size_t)(j0 + u * G::TL) * st];
+  }
   // twiddle loads with the data, the line table after them
   D2 w{1, 0}, st16{1, 0};
   if constexpr (TW != 0) {
@@ -324,22 +362,25 @@ hipError_t pass1(bool fwd, const F1Args& a, hipStream_t s, const void* in, void*
   }
 }
 
-template <typename R, int L, bool INV, int TW>
-hipError_t go_mid(const F1Args& a, hipStream_t s, void* grid, int st, int ystr) {
+template <typename R, int L, bool INV, int TW, bool FOLD = false>
+hipError_t go_mid(const F1Args& a, const KArgs* k, hipStream_t s, void* grid, int st, int ystr) {
   using C = typename cpx<R>::T;
   using G = LineGeo<R, L>;
   if (a.n1 % G::LPB) return hipErrorInvalidValue;
+  KArgs kk;
+  if (k) kk = *k;
+  else std::memset(&kk, 0, sizeof kk);
   const dim3 g((unsigned)(a.n1 / G::LPB), (unsigned)a.Lm), b(G::NT);
-  hipLaunchKernelGGL((k_fft1_mid<R, L, INV, TW>), g, b, 0, s, a, (C*)grid, st, ystr);
+  hipLaunchKernelGGL((k_fft1_mid<R, L, INV, TW, FOLD>), g, b, 0, s, a, kk, (C*)grid, st, ystr);
   return hipGetLastError();
 }
 
-template <typename R, bool INV, int TW>
-hipError_t mid_l(const F1Args& a, hipStream_t s, void* grid, int st, int ystr) {
+template <typename R, bool INV, int TW, bool FOLD = false>
+hipError_t mid_l(const F1Args& a, const KArgs* k, hipStream_t s, void* grid, int st, int ystr) {
   switch (a.Lm) {
-    case 64: return go_mid<R, 64, INV, TW>(a, s, grid, st, ystr);
-    case 128: return go_mid<R, 128, INV, TW>(a, s, grid, st, ystr);
-    case 256: return go_mid<R, 256, INV, TW>(a, s, grid, st, ystr);
+    case 64: return go_mid<R, 64, INV, TW, FOLD>(a, k, s, grid, st, ystr);
+    case 128: return go_mid<R, 128, INV, TW, FOLD>(a, k, s, grid, st, ystr);
+    case 256: return go_mid<R, 256, INV, TW, FOLD>(a, k, s, grid, st, ystr);
     default: return hipErrorInvalidValue;
   }
 }
@@ -348,17 +389,21 @@ template <typename R>
 hipError_t fwd_rest(const F1Args& a, hipStream_t s, void* grid) {
   const int n12 = a.n1 * a.Lm;
   // pass 2: lines (a, j3), points j2 at stride n1, line offset j3 n1 n2
-  hipError_t e = mid_l<R, false, 1>(a, s, grid, a.n1, n12);
+  hipError_t e = mid_l<R, false, 1>(a, nullptr, s, grid, a.n1, n12);
   // pass 3: lines (a, k2), points j3 at stride n1 n2, line offset k2 n1
-  if (e == hipSuccess) e = mid_l<R, false, 0>(a, s, grid, n12, a.n1);
+  if (e == hipSuccess) e = mid_l<R, false, 0>(a, nullptr, s, grid, n12, a.n1);
   return e;
 }
 
+// Adjoint passes 3 and 2 (part 0), pass 3 with the fold of the spreading
+// slots (part 3), pass 2 alone (part 2).
 template <typename R>
-hipError_t adj_first(const F1Args& a, hipStream_t s, void* grid) {
+hipError_t adj_first(const F1Args& a, const KArgs* k, hipStream_t s, void* grid, int part) {
   const int n12 = a.n1 * a.Lm;
-  hipError_t e = mid_l<R, true, 2>(a, s, grid, n12, a.n1);
-  if (e == hipSuccess) e = mid_l<R, true, 3>(a, s, grid, a.n1, n12);
+  hipError_t e = hipSuccess;
+  if (part == 0) e = mid_l<R, true, 2>(a, nullptr, s, grid, n12, a.n1);
+  if (part == 3) return mid_l<R, true, 2, true>(a, k, s, grid, n12, a.n1);
+  if (e == hipSuccess) e = mid_l<R, true, 3>(a, nullptr, s, grid, a.n1, n12);
   return e;
 }
 
@@ -390,8 +435,14 @@ hipError_t launch_fft1_fwd(bool dbl, const F1Args& a, hipStream_t s, const void*
 }
 
 hipError_t launch_fft1_adj(bool dbl, const F1Args& a, hipStream_t s, void* grid, void* f, int part) {
-  if (part == 0) return dbl ? adj_first<double>(a, s, grid) : adj_first<float>(a, s, grid);
-  return dbl ? pass1<double>(false, a, s, grid, f) : pass1<float>(false, a, s, grid, f);
+  if (part == 1) return dbl ? pass1<double>(false, a, s, grid, f) : pass1<float>(false, a, s, grid, f);
+  if (part != 0 && part != 2) return hipErrorInvalidValue;
+  return dbl ? adj_first<double>(a, nullptr, s, grid, part) : adj_first<float>(a, nullptr, s, grid, part);
+}
+
+hipError_t launch_fft1_adj_fold(bool dbl, const F1Args& a, const KArgs& k, hipStream_t s, void* grid) {
+  if (k.D != 1) return hipErrorInvalidValue;
+  return dbl ? adj_first<double>(a, &k, s, grid, 3) : adj_first<float>(a, &k, s, grid, 3);
 }
 
 }  // namespace nfftm

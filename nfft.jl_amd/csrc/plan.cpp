@@ -1018,10 +1018,14 @@ struct ProfScope {
   }
 };
 
-// The 3-D adjoint's fold runs inside its first pruned FFT pass (see
-// launch_fold_fft3) unless NFFTM_FLAG_SEPARATE_FOLD is set.
+// The adjoint's fold runs inside its first FFT pass unless
+// NFFTM_FLAG_SEPARATE_FOLD is set: 3-D plans on the pruned passes (see
+// launch_fold_fft3) and tiled 1-D plans on the three-pass FFT (see
+// launch_fft1_adj_fold).
 bool fused_fold(const nfftm_plan_s& p) {
-  return p.pfft && p.blocked3() && p.single_items && !(p.flags & NFFTM_FLAG_SEPARATE_FOLD) &&
+  if (p.flags & NFFTM_FLAG_SEPARATE_FOLD) return false;
+  if (p.f1) return p.tiled && p.D == 1;
+  return p.pfft && p.blocked3() && p.single_items &&
          launch_fold_fft3(p.dbl(), p.W, p.kargs(), p.fargs(), nullptr, nullptr, true) == hipSuccess;
 }
 
@@ -1171,9 +1175,13 @@ int grid_to_image(nfftm_plan_s& p, void* f, Mark&& mark, bool fused = false) {
   }
   if (p.f1) {
     const F1Args a = p.f1args();
+    if (fused) {
+      ProfScope ps(p, 3);
+      HIPCHK(launch_fft1_adj_fold(p.dbl(), a, p.kargs(), p.stream, p.d_grid.p));
+    }
     {
       ProfScope ps(p, 5);
-      HIPCHK(launch_fft1_adj(p.dbl(), a, p.stream, p.d_grid.p, f, 0));
+      HIPCHK(launch_fft1_adj(p.dbl(), a, p.stream, p.d_grid.p, f, fused ? 2 : 0));
     }
     if (int rc = mark()) return rc;
     ProfScope ps(p, 6);

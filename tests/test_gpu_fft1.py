@@ -129,3 +129,37 @@ def test_fft1_not_used_off_the_factorisations():
     f = torch.as_tensor(crandn(rng, (N,)), dtype=torch.complex128, device="cuda")
     orc = O.OraclePlan(k, (N,), m=4, sigma=Nt / N, precompute=O.LINEAR)
     assert rel(p * f, orc.forward(f.cpu().numpy())) < 1e-12
+
+
+FOLD_CASES = [(nfftm.TENSOR, torch.float32), (nfftm.TENSOR, torch.float64),
+              (nfftm.LINEAR, torch.float32), (nfftm.LINEAR, torch.float64)]
+
+
+@pytest.mark.parametrize("pre,dtype", FOLD_CASES, ids=["tensor-f32", "tensor-f64", "linear-f32", "linear-f64"])
+def test_fft1_fused_fold_clustered(pre, dtype):
+    """The adjoint's fold fused into the first FFT pass against the separate
+    fold (``fused_fold=False``) and the rocFFT path, on nodes clustered at
+    the ends of the period (Beta(1/2, 1/2), as the contention-stress
+    workload) so that end tiles hold many work items and most points near
+    them take the general fold; FP64 <= 1e-13 between the paths and
+    <= 1e-12 against the oracle, FP32 <= 2e-6 between the paths."""
+    print(f"fft1 fused fold {pre} {dtype}", flush=True)
+    rng = np.random.default_rng([11, pre, dtype == torch.float64])
+    N, Nt, M = 1 << 20, 1310720, 400000
+    k = (rng.beta(0.5, 0.5, (1, M)) - 0.5)
+    kw = dict(m=4, sigma=Nt / N, precompute=pre, dtype=dtype)
+    p = nfftm.plan_nfft(k, (N,), **kw)
+    q = nfftm.plan_nfft(k, (N,), fused_fold=False, **kw)
+    r = nfftm.plan_nfft(k, (N,), rocfft=True, **kw)
+    assert p.Ntilde == (Nt,)
+    cd = torch.complex128 if dtype == torch.float64 else torch.complex64
+    tol = 1e-13 if dtype == torch.float64 else 2e-6
+    for it in range(2):
+        fh = torch.as_tensor(crandn(rng, M), dtype=cd, device="cuda")
+        x, xq, xr = p.H * fh, q.H * fh, r.H * fh
+        torch.cuda.synchronize()
+        assert rel(x, xq) < tol, it
+        assert rel(x, xr) < tol, it
+    if dtype == torch.float64 and pre == nfftm.LINEAR:
+        orc = O.OraclePlan(k, (N,), m=4, sigma=Nt / N, precompute=O.LINEAR)
+        assert rel(x, orc.adjoint(fh.cpu().numpy())) < 1e-12
