@@ -89,7 +89,10 @@ enum {
                               are one kernel where every tile holds at most
                               one work item and the tile shape matches the
                               pass (e.g. 256-point z lines): bit-identical
-                              results, the folded grid is never written. */
+                              results, the folded grid is never written.
+                              Also for tiled 1-D plans on the three-pass
+                              FFT: their fold runs inside its first adjoint
+                              pass. */
   NFFTM_FLAG_SPARSE = 32    /* convolution as a device sparse matrix, the
                               reference GPU backend's form of precompute =
                               FULL (ext/NFFTGPUArraysExt): at nodes! the plan
