@@ -90,7 +90,11 @@ __global__ __launch_bounds__(NT1) void k_interp1p(KArgs a, const typename cpx<R>
                                                   typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
   extern __shared__ __align__(16) char smem[];
-  const int item = xcd_item(a.nitems);
+  // consecutive items on consecutive workgroups, i.e. round robin over the
+  // XCDs: items of clustered nodes are full and sparse regions' items are
+  // not, and contiguous item ranges per XCD left some XCDs with several
+  // times the nodes of others
+  const int item = blockIdx.x;
   if (item >= a.nitems) return;
   const int4 it = a.items[item];
   C* tile = (C*)smem;
@@ -132,7 +136,7 @@ template <typename R, int W>
 __global__ __launch_bounds__(NT1) void k_spread1p(KArgs a, const typename cpx<R>::T* __restrict__ fhat) {
   using C = typename cpx<R>::T;
   extern __shared__ __align__(16) char smem[];
-  const int item = xcd_item(a.nitems);
+  const int item = blockIdx.x;  // round robin over the XCDs (k_interp1p)
   if (item >= a.nitems) return;
   const int4 it = a.items[item];
   double* acc = (double*)smem;  // re plane [0, te), im plane [te, 2 te)
