@@ -52,8 +52,8 @@ F32_TOL = 1e-5  # rel-l2 bar of every FP32 result (against the FP64 plan and the
 # when those sources change, the figure is reported as stale (traffic null)
 # until it is measured again.
 TRAFFIC_SOURCES = ("k_tile3.hip", "kernels.hpp", "launch.hpp", "fft_line.hpp")
-MEASURED_TRAFFIC = {("3d", "spreading"): (1087500300, "b2ed3755f0af1ff5"),
-                    ("3d", "interpolation"): (644703320, "b2ed3755f0af1ff5")}
+MEASURED_TRAFFIC = {("3d", "spreading"): (1086917875, "53e2f819487db85e"),
+                    ("3d", "interpolation"): (644968774, "53e2f819487db85e")}
 
 # Stage order of nfftm_profile_read.  "fold": the per-item padded tiles summed
 # into the grid; on 3-D power-of-two grids with one work item per tile it is

@@ -590,7 +590,10 @@ __device__ __forceinline__ void spread_tile_item(const KArgs& a, int item,
 template <typename R, int D, int W, int NT>
 __global__ __launch_bounds__(NT) void k_spread_tile(KArgs a, const typename cpx<R>::T* __restrict__ fhat,
                                                     typename cpx<R>::T* __restrict__ unused) {
-  const int item = xcd_item(a.nitems);
+  // items round robin over the XCDs: where nodes cluster, the items of one
+  // region are much heavier than the rest, and contiguous item ranges per
+  // XCD (xcd_item) leave the XCDs unevenly loaded
+  const int item = blockIdx.x;
   if (item >= a.nitems) return;
   spread_tile_item<R, D, W, NT>(a, item, fhat, (typename cpx<R>::T*)a.scratch);
 }
@@ -751,7 +754,7 @@ __device__ __forceinline__ void interp_tile_item(const KArgs& a, int item,
 template <typename R, int D, int W, int NT>
 __global__ __launch_bounds__(NT) void k_interp_tile(KArgs a, const typename cpx<R>::T* __restrict__ grid,
                                                     typename cpx<R>::T* __restrict__ fhat) {
-  const int item = xcd_item(a.nitems);
+  const int item = blockIdx.x;  // round robin over the XCDs (k_spread_tile)
   if (item >= a.nitems) return;
   interp_tile_item<R, D, W, NT>(a, item, grid, fhat);
 }
