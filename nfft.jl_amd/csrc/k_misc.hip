@@ -495,18 +495,24 @@ hipError_t launch_inv_perm(const int* perm, int* iperm, long long n, hipStream_t
 // caches this is cheaper than the sorted kernel storing at the caller's
 // positions (8-byte stores at random positions cost a partial-line write
 // each; measured on 2^24 nodes: 337 us of stores against ~180 us here).
+// Each XCD takes one contiguous range of outputs (the dispatcher deals
+// workgroups to the 8 XCDs round robin; gridDim.x is a multiple of 8): the
+// outputs of one range read, in every tile, one contiguous part of its run,
+// one line at a time, so that each line is fetched into one L2 and used
+// there by all its outputs, rather than by every XCD's L2.
 template <typename C>
 __global__ __launch_bounds__(256) void k_gather_nodes(const C* __restrict__ in, long long is,
                                                       const int* __restrict__ idx, C* __restrict__ out,
                                                       long long os, long long n) {
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+  const long long b = (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  for (long long i = b * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
     out[i * os] = in[(long long)idx[i] * is];
 }
 
 hipError_t launch_gather_nodes(bool dbl, const void* in, long long is, const int* idx, void* out, long long os,
                                long long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+  const unsigned blocks = (unsigned)(8 * ((std::min<long long>((n + 255) / 256, 65536) + 7) / 8));
   if (dbl)
     hipLaunchKernelGGL(k_gather_nodes<double2>, dim3(blocks), dim3(256), 0, s, (const double2*)in, is, idx,
                        (double2*)out, os, n);
