@@ -163,3 +163,24 @@ def test_fft1_fused_fold_clustered(pre, dtype):
     if dtype == torch.float64 and pre == nfftm.LINEAR:
         orc = O.OraclePlan(k, (N,), m=4, sigma=Nt / N, precompute=O.LINEAR)
         assert rel(x, orc.adjoint(fh.cpu().numpy())) < 1e-12
+
+
+@pytest.mark.parametrize("pre", [nfftm.LINEAR, nfftm.TENSOR], ids=["linear", "tensor"])
+def test_fft1_empty_nodes(pre):
+    """M = 0 on a three-pass grid: the forward is empty and the adjoint, with
+    its fold inside the first FFT pass (tiles without work items), is zero;
+    nodes! then brings nodes back and the plan matches the rocFFT path."""
+    print(f"fft1 empty nodes {pre}", flush=True)
+    N, Nt = 1 << 20, 1310720
+    p = nfftm.plan_nfft(np.zeros((1, 0)), (N,), m=4, sigma=Nt / N, precompute=pre, dtype=torch.float32)
+    assert p.Ntilde == (Nt,)
+    y = p * torch.ones(N, dtype=torch.complex64, device="cuda")
+    assert y.shape == (0,)
+    x = p.H * torch.zeros(0, dtype=torch.complex64, device="cuda")
+    assert float(x.abs().max()) == 0.0
+    rng = np.random.default_rng(5)
+    k = rng.random((1, 3000)) - 0.5
+    p.nodes_(k)
+    q = make(k, N, Nt, torch.float32, True, pre)
+    fh = torch.as_tensor(crandn(rng, 3000), dtype=torch.complex64, device="cuda")
+    assert rel(p.H * fh, q.H * fh) < 2e-6
