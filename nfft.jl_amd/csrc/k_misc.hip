@@ -500,19 +500,32 @@ hipError_t launch_inv_perm(const int* perm, int* iperm, long long n, hipStream_t
 // outputs of one range read, in every tile, one contiguous part of its run,
 // one line at a time, so that each line is fetched into one L2 and used
 // there by all its outputs, rather than by every XCD's L2.
+// Four outputs per thread, their index loads and then their gathers issued
+// together (one output per thread left the kernel waiting on two dependent
+// loads per output).
 template <typename C>
 __global__ __launch_bounds__(256) void k_gather_nodes(const C* __restrict__ in, long long is,
                                                       const int* __restrict__ idx, C* __restrict__ out,
                                                       long long os, long long n) {
+  constexpr int U = 4;
   const long long b = (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-  for (long long i = b * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    out[i * os] = in[(long long)idx[i] * is];
+  for (long long i0 = b * 256 * U + threadIdx.x; i0 < n; i0 += (long long)gridDim.x * 256 * U) {
+    int j[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) j[u] = i0 + u * 256 < n ? idx[i0 + u * 256] : -1;
+    C v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = in[(long long)(j[u] < 0 ? 0 : j[u]) * is];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j[u] >= 0) out[(i0 + u * 256) * os] = v[u];
+  }
 }
 
 hipError_t launch_gather_nodes(bool dbl, const void* in, long long is, const int* idx, void* out, long long os,
                                long long n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)(8 * ((std::min<long long>((n + 255) / 256, 65536) + 7) / 8));
+  const unsigned blocks = (unsigned)(8 * ((std::min<long long>((n + 1023) / 1024, 65536) + 7) / 8));
   if (dbl)
     hipLaunchKernelGGL(k_gather_nodes<double2>, dim3(blocks), dim3(256), 0, s, (const double2*)in, is, idx,
                        (double2*)out, os, n);
